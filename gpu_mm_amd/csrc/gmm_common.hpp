@@ -1,0 +1,182 @@
+// Geometry, error bits and plan-entry format shared by the host code and the gfx950 kernels.
+//
+// Everything here is __host__ __device__ so that the CPU "reference" operators, the planner
+// test aid and the HIP kernels locate a sample with the same IEEE float operations, and
+// therefore agree bit for bit on which pixels and cells a sample touches.
+
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace gmm {
+
+// ------------------------------------------------------------------------------------------
+// Error bits, reported per block by the kernels and turned into exceptions by the host.
+// Values follow include/gpu_mm_internals.hpp:26-29 (same meaning, same messages).
+
+constexpr uint32_t ERR_BAD_YPIX = 0x1;
+constexpr uint32_t ERR_BAD_XPIX = 0x2;
+constexpr uint32_t ERR_INCONSISTENT_NMT = 0x4;
+constexpr uint32_t ERR_NOT_IN_PIXELIZATION = 0x8;
+
+// ------------------------------------------------------------------------------------------
+// Plan entry ("mt") bit layout, src_lib/PointingPlan.cu:75-89 and include/gpu_mm.hpp:120-129:
+//   bits [0,10)  ixcell        (global x cell index)
+//   bits [10,20) iycell        (global y cell index)
+//   bits [20,46) icl           (TOD cache line = sample index >> 5)
+//   bit  46      mflag         (the cache line touches more than one cell)
+//   bit  47      zflag         (exactly one entry per multi-cell line carries it: the entry
+//                               whose cell holds the (iy0,ix0) corner of the line's first sample)
+
+constexpr int CELL_BITS = 20;
+constexpr uint32_t CELL_MASK = (1u << 20) - 1;
+constexpr int ICL_SHIFT = 20;
+constexpr uint32_t ICL_MASK = (1u << 26) - 1;
+constexpr uint64_t MFLAG = 1ull << 46;
+constexpr uint64_t ZFLAG = 1ull << 47;
+
+// A cell is 64x64 pixels; one cell of a T/Q/U map holds 3*64*64 floats (48 KiB).
+constexpr int CELL_SIDE = 64;
+constexpr int CELL_NPIX = 64 * 64;
+
+// Samples per plan entry (one TOD cache line of float32).
+constexpr int LINE = 32;
+
+// Preplan block count (include/gpu_mm.hpp:63).
+constexpr int PREPLAN_SIZE = 1024;
+
+__host__ __device__ inline uint32_t cell_key(int iycell, int ixcell)
+{
+    return (uint32_t(iycell) << 10) | uint32_t(ixcell);
+}
+
+__host__ __device__ inline int64_t align128(int64_t n) { return (n + 127) & ~int64_t(127); }
+
+// ------------------------------------------------------------------------------------------
+// Pixel location. Semantics follow pixel_locator<T>::locate, include/gpu_mm_internals.hpp:124-164:
+//
+//   - y: iy0 = clamp(trunc(y), 0, ny-2), iy1 = iy0+1, dy = y - iy0; valid iff 0 <= y <= ny-1.
+//   - non-periodic x: the same with nx; valid iff 0 <= x <= nx-2.
+//   - periodic x: valid iff -nx <= x <= 2nx. One conditional shift by +nx (x < 0) and one by
+//     -nx (x > nx) brings x into [-eps, nx+eps]; ix0 = max(trunc(x),0), dx = x - ix0, then
+//     ix0 == nx is folded to 0 and ix1 = ix0+1 wraps to 0 at nx.
+//
+// The arithmetic is a handful of compares, conversions and exact subtractions, plus the two
+// float additions x +- nx, so host and device results agree exactly (no fused operations).
+// Out-of-range inputs still produce in-range indices; the returned error bits flag them.
+
+template<typename T>
+struct Pixel
+{
+    int iy0, iy1, ix0, ix1;
+    T dy, dx;
+};
+
+template<typename T>
+__host__ __device__ inline int trunc_to_int(T v)
+{
+    // NaN and values beyond the int range give a defined (clamped) result on host and device;
+    // such samples are always flagged as bad by the range checks.
+    if (!(v > T(-2147483520.0)))
+        return (v == v) ? -2147483647 : 0;
+    if (!(v < T(2147483520.0)))
+        return 2147483647;
+    return int(v);
+}
+
+template<typename T>
+__host__ __device__ inline uint32_t locate_pixel(T y, T x, int ny, int nx, bool periodic, Pixel<T> &p)
+{
+    uint32_t err = 0;
+
+    int iy = trunc_to_int(y);
+    iy = (iy > 0) ? iy : 0;
+    iy = (iy < ny - 2) ? iy : (ny - 2);
+    p.iy0 = iy;
+    p.iy1 = iy + 1;
+    p.dy = y - T(iy);
+    if (!((y >= T(0)) && (y <= T(ny - 1))))
+        err |= ERR_BAD_YPIX;
+
+    if (periodic) {
+        if (!((x >= T(-nx)) && (x <= T(2 * nx))))
+            err |= ERR_BAD_XPIX;
+        x = (x >= T(0)) ? x : (x + T(nx));
+        x = (x <= T(nx)) ? x : (x - T(nx));
+        int ix = trunc_to_int(x);
+        ix = (ix > 0) ? ix : 0;
+        p.dx = x - T(ix);
+        ix = (ix < nx) ? ix : 0;
+        p.ix0 = ix;
+        p.ix1 = (ix + 1 < nx) ? (ix + 1) : 0;
+    }
+    else {
+        if (!((x >= T(0)) && (x <= T(nx - 2))))
+            err |= ERR_BAD_XPIX;
+        int ix = trunc_to_int(x);
+        ix = (ix > 0) ? ix : 0;
+        ix = (ix < nx - 2) ? ix : (nx - 2);
+        p.ix0 = ix;
+        p.ix1 = ix + 1;
+        p.dx = x - T(ix);
+    }
+
+    return err;
+}
+
+// The (up to four) distinct cells touched by the bilinear footprint of one sample, as
+// cell keys; duplicates within the sample are replaced by NO_CELL.
+constexpr uint32_t NO_CELL = 0xffffffffu;
+
+template<typename T>
+__host__ __device__ inline void footprint_cells(const Pixel<T> &p, uint32_t k[4])
+{
+    int cy0 = p.iy0 >> 6, cy1 = p.iy1 >> 6;
+    int cx0 = p.ix0 >> 6, cx1 = p.ix1 >> 6;
+    bool ysplit = (cy1 != cy0);
+    bool xsplit = (cx1 != cx0);
+    k[0] = cell_key(cy0, cx0);
+    k[1] = xsplit ? cell_key(cy0, cx1) : NO_CELL;
+    k[2] = ysplit ? cell_key(cy1, cx0) : NO_CELL;
+    k[3] = (xsplit && ysplit) ? cell_key(cy1, cx1) : NO_CELL;
+}
+
+// ------------------------------------------------------------------------------------------
+// Launch-parameter formulas, src_lib/PointingPrePlan.cu:162-168 and 236-250. They define the
+// planner block partition (and therefore the nmt_cumsum array that the tests compare), so
+// they are kept identical.
+
+struct PreplanParams
+{
+    int64_t ncl;                  // number of 32-sample lines
+    int64_t ncl_per_threadblock;  // lines per planner block
+    int64_t planner_nblocks;
+};
+
+__host__ inline PreplanParams preplan_params(int64_t nsamp)
+{
+    PreplanParams r;
+    r.ncl = (nsamp + 31) / 32;
+    int64_t a = (r.ncl + PREPLAN_SIZE - 1) / PREPLAN_SIZE;
+    r.ncl_per_threadblock = (a > 16) ? a : 16;
+    r.ncl_per_threadblock = (r.ncl_per_threadblock + 3) & ~int64_t(3);
+    r.planner_nblocks = (r.ncl + r.ncl_per_threadblock - 1) / r.ncl_per_threadblock;
+    return r;
+}
+
+// ------------------------------------------------------------------------------------------
+// Operator work unit: a contiguous run of plan entries that all belong to one cell.
+// The sorted plan is cut into these at plan time; each map2tod/tod2map block takes one.
+
+struct Chunk
+{
+    uint32_t icell;   // cell key (iycell << 10 | ixcell)
+    uint32_t begin;   // first plan entry
+    uint32_t end;     // one past the last plan entry
+    uint32_t flags;   // CHUNK_SHARED: the cell's entries are split over more than one chunk
+};
+
+constexpr uint32_t CHUNK_SHARED = 1u;
+
+}  // namespace gmm

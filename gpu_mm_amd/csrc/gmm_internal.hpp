@@ -1,0 +1,171 @@
+// Internal definitions of libgpu_mm_amd.so: handle structs, error plumbing, launchers.
+
+#pragma once
+
+#include "gmm_common.hpp"
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+// Opaque C-ABI handles (declared in include/gpu_mm_amd.h).
+
+struct gmm_lpix
+{
+    int64_t nypix_global = 0;
+    int64_t nxpix_global = 0;
+    bool periodic_xcoord = true;
+    const int64_t *offsets_cpu = nullptr;  // caller-owned int64[nycells][nxcells]
+    const int64_t *offsets_gpu = nullptr;  // caller-owned device copy
+    int64_t nycells = 0, nxcells = 0;
+    int64_t ystride = 0, polstride = 0;
+    int64_t npix = 0;
+};
+
+struct gmm_preplan
+{
+    int64_t nsamp = 0, nypix_global = 0, nxpix_global = 0;
+    bool periodic_xcoord = true;
+    gmm::PreplanParams pparams{};
+    uint32_t *nmt_cumsum_gpu = nullptr;   // caller-owned uint32[PREPLAN_SIZE]
+
+    int64_t plan_nmt = 0;
+    double overhead = 0.0;
+    int64_t nmt_per_threadblock = 0, pointing_nblocks = 0;
+
+    // Derived sizes (computed once in the constructor).
+    int64_t nycells_g = 0, nxcells_g = 0;  // cells of the global map
+    int key_bits = 0;                      // bits of the dense cell key iycell*nxcells_g + ixcell
+    int64_t chunk_cap = 0;                 // max plan entries per operator work unit
+    int64_t max_nchunks = 0;
+    int64_t max_nmlines = 0;
+    int64_t plan_nbytes = 0, tmp_nbytes = 0, sort_nbytes = 0;
+
+    // Offsets inside the plan buffer.
+    int64_t off_err = 0, off_chunks = 0, off_mlines = 0, n_err = 0;
+};
+
+struct gmm_plan
+{
+    gmm_preplan pp;                 // copy (small)
+    uint8_t *buf = nullptr;         // caller-owned device buffer, >= pp.plan_nbytes
+    uint64_t *plan_mt = nullptr;    // sorted plan entries
+    uint32_t *err = nullptr;        // per-block error words (n_err of them)
+    gmm::Chunk *chunks = nullptr;   // operator work list
+    uint32_t *mlines = nullptr;     // lines that touch more than one cell
+    int64_t nchunks = 0;
+    int64_t nmlines = 0;
+};
+
+namespace gmm {
+
+struct Error : public std::runtime_error
+{
+    explicit Error(const std::string &s) : std::runtime_error(s) { }
+};
+
+[[noreturn]] void fail(const std::string &msg);
+void hip_check(hipError_t e, const char *what);
+
+#define GMM_HIP(call) ::gmm::hip_check((call), #call)
+#define GMM_REQUIRE(cond, msg) do { if (!(cond)) ::gmm::fail(std::string(msg)); } while (0)
+
+// Converts a set of error bits into the exception the reference raises
+// (src_lib/check_arguments.cu:159-175).
+void throw_errflags(uint32_t err, const char *where, uint32_t ignore = 0);
+
+// Argument checks (src_lib/check_arguments.cu:137-156).
+void check_nsamp(int64_t nsamp, const char *where);
+void check_npix_global(int64_t n, const char *what, const char *where);
+
+// ------------------------------------------------------------------------------------------
+// Device scan (exclusive, in place) over uint32. Scratch size from scan_scratch_nbytes().
+
+int64_t scan_scratch_nbytes(int64_t n);
+void exclusive_scan_u32(uint32_t *data, int64_t n, void *scratch, hipStream_t stream);
+
+// ------------------------------------------------------------------------------------------
+// Planner launchers (k_planner.hip).
+
+void launch_preplan_count(const float *xp, int64_t nsamp, const gmm_preplan &pp,
+                          uint32_t *nmt_out, uint32_t *err_out, hipStream_t stream);
+
+// Inclusive scan of nmt[0:nblocks] in place, padding to PREPLAN_SIZE with the total, and
+// OR of err[0:nblocks] into err[PREPLAN_SIZE-1]... returned through 'summary_gpu' (2 words:
+// total nmt, OR of error bits).
+void launch_preplan_finish(uint32_t *nmt, const uint32_t *err, int64_t nblocks,
+                           uint32_t *summary_gpu, hipStream_t stream);
+
+void launch_plan_emit(const float *xp, const gmm_preplan &pp, uint64_t *out_mt,
+                      uint32_t *err_out, hipStream_t stream);
+
+// Sorts plan entries by cell (stable), ping-ponging between a and b, result in 'out'.
+void plan_sort(const gmm_preplan &pp, uint64_t *a, uint64_t *b, uint64_t *out,
+               void *scratch, int64_t scratch_nbytes, hipStream_t stream);
+
+// Builds the chunk list and the multi-cell line list from the sorted plan. Writes the
+// two counts to counts_gpu[0], counts_gpu[1].
+void plan_schedule(const gmm_preplan &pp, const uint64_t *sorted_mt, Chunk *chunks,
+                   uint32_t *mlines, uint32_t *counts_gpu, void *scratch,
+                   int64_t scratch_nbytes, hipStream_t stream);
+
+// Size of the scratch used by plan_sort() + plan_schedule().
+int64_t plan_scratch_nbytes(const gmm_preplan &pp);
+
+// Fills n words with v (a kernel, so it is safe inside captured graphs).
+void launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t stream);
+
+// OR-reduce n words into out[0].
+void launch_reduce_or(const uint32_t *in, int64_t n, uint32_t *out, hipStream_t stream);
+
+// Quantize kernel for the plan tester (src_lib/PointingPlanTester.cu:23-63).
+void launch_quantize(const float *xp, int64_t nsamp, int ny, int nx, bool periodic,
+                     int32_t *iypix, int32_t *ixpix, uint32_t *err_word, hipStream_t stream);
+
+// Work-list coverage test (src_lib/test_plan_iterator.cu:52-100 role).
+void launch_iterator_test(const uint64_t *plan_mt, const Chunk *chunks, int64_t nchunks,
+                          int waves_per_block, uint32_t *visit_counts, uint32_t *err_word,
+                          hipStream_t stream);
+
+// Builds chunks from an arbitrary cell-sorted plan (used by the iterator test).
+int64_t build_chunks_generic(const uint64_t *plan_mt_gpu, int64_t nmt, int64_t chunk_cap,
+                             Chunk *chunks_gpu, int64_t max_chunks, void *scratch,
+                             int64_t scratch_nbytes, hipStream_t stream);
+int64_t build_chunks_generic_scratch(int64_t nmt);
+
+// ------------------------------------------------------------------------------------------
+// Operator launchers (k_planned.hip, k_unplanned.hip).
+
+struct OpArgs
+{
+    const gmm_lpix *lpix;
+    int64_t nsamp;
+    bool partial;
+};
+
+void launch_planned_map2tod(float *tod, const float *lmap, const float *xp, const OpArgs &a,
+                            const gmm_plan &plan, bool accum, hipStream_t stream);
+void launch_planned_tod2map(float *lmap, const float *tod, const float *xp, const OpArgs &a,
+                            const gmm_plan &plan, hipStream_t stream);
+
+int64_t unplanned_nblocks(int64_t nsamp, int64_t max_blocks);
+void launch_unplanned_map2tod(float *tod, const float *lmap, const float *xp, const OpArgs &a,
+                              uint32_t *errflags, int64_t nblocks, bool accum, hipStream_t stream);
+void launch_unplanned_tod2map(float *lmap, const float *tod, const float *xp, const OpArgs &a,
+                              uint32_t *errflags, int64_t nblocks, hipStream_t stream);
+
+// ------------------------------------------------------------------------------------------
+// Host code (host_ref.cpp).
+
+void reference_map2tod(float *tod, const float *lmap, const float *xp, const OpArgs &a, bool accum);
+void reference_tod2map(float *lmap, const float *tod, const float *xp, const OpArgs &a);
+double toy_pointing_fill(float *xp, int64_t nsamp, int64_t ny, int64_t nx, double scan_speed,
+                         double total_drift);
+void plan_tester_host(const gmm_preplan &pp, const int32_t *iypix, const int32_t *ixpix,
+                      uint32_t *nmt_cumsum, uint64_t *sorted_mt);
+std::vector<uint64_t> make_random_plan_mt(int64_t ncells, int64_t min_nmt, int64_t max_nmt,
+                                          uint64_t seed);
+
+}  // namespace gmm

@@ -1,0 +1,441 @@
+"""
+gpu_mm_amd: planned pointing operators for CMB map-making on AMD MI355X (gfx950).
+
+Drop-in for the pointing-operator part of kmsmith137/gpu_mm (gpu_mm/gpu_mm.py). The data model
+is the reference's; only the array library changes: GPU arrays are torch tensors (on the ROCm
+device), CPU arrays are numpy arrays.
+
+  map2tod(tod, local_map, xpointing, plan, partial_pixelization=False, response=None, debug=False, accum=False)
+  tod2map(local_map, tod, xpointing, plan, partial_pixelization=False, response=None, debug=False)
+
+map2tod() overwrites its output TOD (or adds to it, accum=True); tod2map() always adds into
+the existing map.
+
+Global maps are float32[3][nypix_global][nxpix_global] with components {T,Q,U}, indexed
+map[y][x]. A timestream is float32[nsamp] or float32[ndet][nt]. Pointing is "exploded":
+xpointing = float32[3][nsamp] (or [3][ndet][nt]) with rows {y, x, alpha}; detectors measure
+T + cos(2 alpha) Q + sin(2 alpha) U. Valid pointing: 0 <= y <= nypix_global-1, and
+0 <= x <= nxpix_global-2 (periodic_xcoord=False) or -nxpix_global <= x <= 2*nxpix_global
+(periodic_xcoord=True).
+
+A local map stores a set of 64x64 "cells" of the global map; element (p, iy, ix) of cell
+(iycell, ixcell) lives at cell_offsets[iycell, ixcell] + p*polstride + iy*ystride + ix, and a
+negative cell offset means "cell not stored". (Reference: gpu_mm/gpu_mm.py:1-240.)
+
+Plans: PointingPrePlan(xpointing, ...) counts the (32-sample line, cell) pairs; PointingPlan
+(preplan, xpointing) lists them, sorted by cell, together with the per-block work list used by
+the planned kernels. A preplan is small and can be kept per TOD; a plan is large and can be
+rebuilt on the fly.
+
+The 'plan' argument of map2tod/tod2map selects the implementation:
+  PointingPlan  -> planned HIP kernels (cells staged in LDS)
+  None          -> plan-free HIP kernels (global gathers / atomics)
+  'reference'   -> single-threaded CPU implementation (host arrays), for testing
+"""
+
+import numpy as np
+import torch
+
+from . import _native
+
+_ext = _native.ext()
+
+# The planned and unplanned operators check the GPU error flags synchronously after every
+# call (as the reference does). set_errflag_check(False) makes calls asynchronous; errors are
+# then accumulated on the plan and raised by PointingPlan.check_errflags().
+_errflag_check = True
+
+mm_dtype = np.float32
+
+
+def set_errflag_check(flag):
+    """Enable (default) or disable the synchronous error-flag check after each GPU operator."""
+    global _errflag_check
+    _errflag_check = bool(flag)
+
+
+def get_errflag_check():
+    return _errflag_check
+
+
+def _stream():
+    if torch.cuda.is_available():
+        return torch.cuda.current_stream().cuda_stream
+    return 0
+
+
+def _device():
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _is_gpu_array(a):
+    return isinstance(a, torch.Tensor) and a.is_cuda
+
+
+def map2tod(tod, local_map, xpointing, plan, partial_pixelization=False, response=None, debug=False, accum=False):
+    """
+    tod (=, or += if accum) A * local_map.
+
+      - tod: float32 array of shape (nsamp,) or (ndet, nt); output.
+      - local_map: LocalMap.
+      - xpointing: float32 array of shape (3, nsamp) or (3, ndet, nt).
+      - plan: PointingPlan, None (plan-free GPU kernels) or 'reference' (CPU, host arrays).
+      - partial_pixelization: if True, pointing outside the local map reads zeros instead of
+        raising.
+      - accum: add into 'tod' instead of overwriting it.
+
+    Reference: gpu_mm/gpu_mm.py:262-312.
+    """
+    if not isinstance(local_map, LocalMap):
+        raise RuntimeError(f"map2tod(): expected LocalMap, got {local_map!r}")
+    lpix = local_map.pixelization
+    larr = local_map.arr
+    if response is not None:
+        raise RuntimeError("map2tod(): per-detector 'response' is not supported by this build")
+
+    if isinstance(plan, PointingPlan):
+        _ext.planned_map2tod(tod, larr, xpointing, lpix, plan, partial_pixelization, debug, accum,
+                             check_errors=_errflag_check, stream=_stream())
+    elif plan is None:
+        errflags = torch.zeros(_UNPLANNED_MAX_BLOCKS, dtype=torch.uint32, device=_device())
+        _ext.unplanned_map2tod(tod, larr, xpointing, lpix, errflags, partial_pixelization, accum,
+                               check_errors=_errflag_check, stream=_stream())
+    elif isinstance(plan, str) and plan == 'reference':
+        _ext.reference_map2tod(tod, larr, xpointing, lpix, partial_pixelization, accum)
+    else:
+        raise RuntimeError(f"Bad 'plan' argument to map2tod(): {plan}")
+
+
+def tod2map(local_map, tod, xpointing, plan, partial_pixelization=False, response=None, debug=False):
+    """
+    local_map += A^T * tod.
+
+    Arguments as in map2tod(); the output map is always accumulated into.
+    Reference: gpu_mm/gpu_mm.py:314-385.
+    """
+    if not isinstance(local_map, LocalMap):
+        raise RuntimeError(f"tod2map(): Bad 'local_map' argument to tod2map(): expected LocalMap, got: {local_map}")
+    lpix = local_map.pixelization
+    larr = local_map.arr
+    if response is not None:
+        raise RuntimeError("tod2map(): per-detector 'response' is not supported by this build")
+
+    if isinstance(plan, PointingPlan):
+        _ext.planned_tod2map(larr, tod, xpointing, lpix, plan, partial_pixelization, debug,
+                             check_errors=_errflag_check, stream=_stream())
+    elif plan is None:
+        errflags = torch.zeros(_UNPLANNED_MAX_BLOCKS, dtype=torch.uint32, device=_device())
+        _ext.unplanned_tod2map(larr, tod, xpointing, lpix, errflags, partial_pixelization,
+                               check_errors=_errflag_check, stream=_stream())
+    elif isinstance(plan, str) and plan == 'reference':
+        _ext.reference_tod2map(larr, tod, xpointing, lpix, partial_pixelization)
+    else:
+        raise RuntimeError(f"Bad 'plan' argument to tod2map(): {plan}")
+
+
+# Block budget of the plan-free kernels (gpu_mm.py:306 uses 2048 as well).
+_UNPLANNED_MAX_BLOCKS = 2048
+
+
+####################################################################################################
+
+
+class LocalPixelization(_ext.LocalPixelization):
+    """
+    Global pixelization (nypix_global, nxpix_global, periodic_xcoord) plus the set of 64x64
+    cells held locally and their memory layout (cell_offsets, ystride, polstride).
+
+    Members: nypix_global, nxpix_global, periodic_xcoord, cell_offsets_cpu (numpy int64),
+    cell_offsets_gpu (torch int64 on the GPU, when a GPU is present), ystride, polstride,
+    nycells, nxcells, npix (local pixels, without the factor 3).
+
+    Reference: gpu_mm/gpu_mm.py:421-526; checks from src_lib/LocalPixelization.cu:33-101.
+    """
+
+    def __init__(self, nypix_global, nxpix_global, cell_offsets, ystride, polstride, periodic_xcoord=True):
+        if isinstance(cell_offsets, torch.Tensor):
+            cpu = cell_offsets.detach().to("cpu").numpy()
+        else:
+            cpu = np.asarray(cell_offsets)
+        self.cell_offsets_cpu = np.ascontiguousarray(cpu, dtype=np.int64)
+        if torch.cuda.is_available():
+            self.cell_offsets_gpu = torch.from_numpy(self.cell_offsets_cpu).to(_device())
+        else:
+            self.cell_offsets_gpu = None
+        _ext.LocalPixelization.__init__(self, int(nypix_global), int(nxpix_global), self.cell_offsets_cpu,
+                                        self.cell_offsets_gpu, int(ystride), int(polstride), bool(periodic_xcoord))
+
+    def is_simple_rectangle(self):
+        """True if the local map is a plain contiguous (3, 64*nycells, 64*nxcells) array."""
+        rect = self._make_rectangular_cell_offsets(self.nycells, self.nxcells)
+        return (self.ystride == 64 * self.nxcells) and (self.polstride == 64 * 64 * self.nycells * self.nxcells) \
+            and np.array_equal(self.cell_offsets_cpu, rect)
+
+    def __eq__(self, x):
+        if not isinstance(x, LocalPixelization):
+            return NotImplemented
+        if self is x:
+            return True
+        if (self.nypix_global, self.nxpix_global) != (x.nypix_global, x.nxpix_global):
+            return False
+        if (self.periodic_xcoord, self.ystride, self.polstride) != (x.periodic_xcoord, x.ystride, x.polstride):
+            return False
+        if (self.nycells, self.nxcells, self.npix) != (x.nycells, x.nxcells, x.npix):
+            return False
+        return bool(np.all(np.maximum(self.cell_offsets_cpu, -1) == np.maximum(x.cell_offsets_cpu, -1)))
+
+    __hash__ = None
+
+    @classmethod
+    def _make_rectangular_cell_offsets(cls, nycells, nxcells):
+        off = np.empty((nycells, nxcells), dtype=np.int64)
+        off[:, :] = 64 * np.arange(nxcells, dtype=np.int64).reshape((1, -1))
+        off[:, :] += 64 * 64 * nxcells * np.arange(nycells, dtype=np.int64).reshape((-1, 1))
+        return off
+
+    @classmethod
+    def make_rectangle(cls, nypix_global, nxpix_global, periodic_xcoord=True):
+        """
+        LocalPixelization covering the whole global map, stored as a contiguous array of
+        shape (3, nypix_padded, nxpix_padded), with n_padded = n rounded up to a multiple of 64.
+        """
+        assert nypix_global > 0
+        assert nxpix_global > 0
+        nycells = (nypix_global + 63) // 64
+        nxcells = (nxpix_global + 63) // 64
+        return LocalPixelization(nypix_global, nxpix_global,
+                                 cell_offsets=cls._make_rectangular_cell_offsets(nycells, nxcells),
+                                 ystride=64 * nxcells, polstride=64 * 64 * nycells * nxcells,
+                                 periodic_xcoord=periodic_xcoord)
+
+    @classmethod
+    def make_cells(cls, nypix_global, nxpix_global, cells, periodic_xcoord=True):
+        """
+        LocalPixelization holding the listed cells (an iterable of (iycell, ixcell)) as a
+        float32[ncells][3][64][64] array, in the order given (ystride=64, polstride=4096).
+        """
+        nycells = (nypix_global + 63) // 64
+        nxcells = (nxpix_global + 63) // 64
+        off = np.full((nycells, nxcells), -1, dtype=np.int64)
+        for i, (cy, cx) in enumerate(cells):
+            if off[cy, cx] >= 0:
+                raise RuntimeError(f"LocalPixelization.make_cells(): cell {(cy, cx)} listed twice")
+            off[cy, cx] = 3 * 64 * 64 * i
+        return LocalPixelization(nypix_global, nxpix_global, off, ystride=64, polstride=64 * 64,
+                                 periodic_xcoord=periodic_xcoord)
+
+
+class LocalMap:
+    """
+    A LocalPixelization plus the array that holds the map: a float32, C-contiguous numpy
+    array (CPU) or torch tensor (GPU) with 3*npix elements. Reference: gpu_mm/gpu_mm.py:529-580.
+    """
+
+    def __init__(self, pixelization, arr):
+        assert isinstance(pixelization, LocalPixelization)
+        if isinstance(arr, torch.Tensor):
+            assert arr.dtype == torch.float32, "LocalMap: expected float32 array"
+            assert arr.is_contiguous(), "LocalMap: expected contiguous array"
+            self.on_gpu = arr.is_cuda
+            self.dtype = np.float32
+        else:
+            assert isinstance(arr, np.ndarray), "LocalMap: expected numpy array or torch tensor"
+            assert arr.dtype == np.float32, "LocalMap: expected float32 array"
+            assert arr.flags['C_CONTIGUOUS'], "LocalMap: expected C-contiguous array"
+            self.on_gpu = False
+            self.dtype = arr.dtype
+
+        self.pixelization = pixelization
+        self.nypix_global = pixelization.nypix_global
+        self.nxpix_global = pixelization.nxpix_global
+        self.periodic_xcoord = pixelization.periodic_xcoord
+        self.npix = pixelization.npix
+        self.arr = arr
+
+    def to_global(self):
+        """
+        Returns the map as a numpy array of shape (3, nypix_global, nxpix_global); cells that
+        are not in the local map are zero. Pixels of boundary cells beyond the global map are
+        ignored (no wrapping). Reference: gpu_mm/gpu_mm.py:565-580, src_lib/local_map_to_global.cu.
+        """
+        if isinstance(self.arr, torch.Tensor):
+            src = self.arr.detach().to("cpu").numpy().reshape(-1)
+        else:
+            src = np.asarray(self.arr).reshape(-1)
+        p = self.pixelization
+        ny, nx = p.nypix_global, p.nxpix_global
+        dst = np.zeros((3, ny, nx), dtype=np.float32)
+        off = p.cell_offsets_cpu
+        iy = np.arange(64).reshape(64, 1)
+        ix = np.arange(64).reshape(1, 64)
+        for cy in range(p.nycells):
+            for cx in range(p.nxcells):
+                o = int(off[cy, cx])
+                if o < 0:
+                    continue
+                y0, x0 = 64 * cy, 64 * cx
+                if y0 >= ny or x0 >= nx:
+                    continue
+                hy, hx = min(64, ny - y0), min(64, nx - x0)
+                for pol in range(3):
+                    idx = o + pol * p.polstride + iy * p.ystride + ix
+                    dst[pol, y0:y0 + hy, x0:x0 + hx] = src[idx][:hy, :hx]
+        return dst
+
+
+####################################################################################################
+
+
+class PointingPrePlan(_ext.PointingPrePlan):
+    """
+    First planning step: counts the (32-sample line, 64x64 cell) pairs of an xpointing array.
+
+    Arguments: xpointing_gpu (float32 (3,nsamp) or (3,ndet,nt) on the GPU), nypix_global,
+    nxpix_global, buf and tmp_buf (uint32[preplan_size] on the GPU, allocated when None; the
+    preplan keeps 'buf', 'tmp_buf' is used only during construction), periodic_xcoord, debug.
+
+    Attributes: nsamp, nypix_global, nxpix_global, periodic_xcoord, plan_nbytes,
+    plan_constructor_tmp_nbytes, overhead, ncl_per_threadblock, planner_nblocks,
+    nmt_per_threadblock, pointing_nblocks, plan_nmt, cub_nbytes (radix-sort scratch bytes),
+    get_nmt_cumsum().
+
+    Reference: gpu_mm/gpu_mm.py:1208-1259; src_lib/PointingPrePlan.cu:129-251.
+    """
+
+    preplan_size = _ext.PointingPrePlan._get_preplan_size()
+
+    def __init__(self, xpointing_gpu, nypix_global, nxpix_global, buf=None, tmp_buf=None, periodic_xcoord=True, debug=False):
+        if buf is None:
+            buf = torch.empty(self.preplan_size, dtype=torch.uint32, device=_device())
+        if tmp_buf is None:
+            tmp_buf = torch.empty(self.preplan_size, dtype=torch.uint32, device=_device())
+        _ext.PointingPrePlan.__init__(self, xpointing_gpu, int(nypix_global), int(nxpix_global), buf, tmp_buf,
+                                      bool(periodic_xcoord), bool(debug), _stream())
+
+
+class PointingPlan(_ext.PointingPlan):
+    """
+    Second planning step: the sorted list of (line, cell) entries plus the work list of the
+    planned kernels.
+
+    Arguments: preplan, xpointing_gpu (the same pointing as the preplan), buf (uint8, at least
+    preplan.plan_nbytes; kept by the plan), tmp_buf (uint8, at least
+    preplan.plan_constructor_tmp_nbytes; used only during construction), debug.
+
+    Attributes: nsamp, nypix_global, nxpix_global, periodic_xcoord, plan_nmt, nchunks,
+    nmlines; get_plan_mt() returns the entries (uint64, bit layout of the reference).
+
+    Reference: gpu_mm/gpu_mm.py:1262-1299; src_lib/PointingPlan.cu:237-312.
+    """
+
+    def __init__(self, preplan, xpointing_gpu, buf=None, tmp_buf=None, debug=False):
+        if buf is None:
+            buf = torch.empty(preplan.plan_nbytes, dtype=torch.uint8, device=_device())
+        if tmp_buf is None:
+            tmp_buf = torch.empty(preplan.plan_constructor_tmp_nbytes, dtype=torch.uint8, device=_device())
+        _ext.PointingPlan.__init__(self, preplan, xpointing_gpu, buf, tmp_buf, bool(debug), _stream())
+
+    def check_errflags(self, where="PointingPlan"):
+        """Raises if an operator call since the last check reported a pointing error."""
+        self._check_errflags(where)
+
+
+####################################################################################################
+
+
+def read_xpointing_npzfile(filename):
+    """
+    Reads an xpointing array from the reference's Nov-2023 .npz format (key 'xpointing', rows
+    ordered (x, y, alpha)); returns float32 (3, ndet, nt_padded) with rows (y, x, alpha) and nt
+    padded to a multiple of 32 by repeating the last sample. Reference: gpu_mm/gpu_mm.py:1305-1329.
+    """
+    with np.load(filename, allow_pickle=False) as f:
+        xp = f['xpointing']
+    assert (xp.ndim == 3) and (xp.shape[0] == 3)
+    ndet, ntu = xp.shape[1], xp.shape[2]
+    ntp = 32 * ((ntu + 31) // 32)
+    out = np.zeros((3, ndet, ntp), dtype=xp.dtype)
+    out[0, :, :ntu] = xp[1, :, :]
+    out[1, :, :ntu] = xp[0, :, :]
+    out[2, :, :ntu] = xp[2, :, :]
+    out[:, :, ntu:] = out[:, :, ntu - 1].reshape((3, -1, 1))
+    return out
+
+
+class ToyPointing:
+    """
+    ToyPointing(ndet, nt, nypix_global, nxpix_global, scan_speed, total_drift, noisy=True):
+    a 45-degree scan bouncing over the full y range with a slow drift in x (test aid).
+
+    Members: xpointing_cpu (numpy), xpointing_gpu (torch, on the GPU when available), ndet, nt,
+    nypix_global, nxpix_global, scan_speed, total_drift, drift_speed.
+    Reference: gpu_mm/gpu_mm.py:1332-1385; src_lib/ToyPointing.cu:41-109.
+    """
+
+    def __init__(self, ndet, nt, nypix_global, nxpix_global, scan_speed, total_drift, noisy=True):
+        shape = (3, ndet, nt) if (ndet is not None) else (3, nt)
+        self.ndet = ndet
+        self.nt = nt
+        self.nypix_global = nypix_global
+        self.nxpix_global = nxpix_global
+        self.scan_speed = scan_speed
+        self.total_drift = total_drift
+        self.xpointing_cpu = np.zeros(shape, dtype=np.float32)
+        if noisy:
+            print(f"{self}: constructor called")
+        self.drift_speed = _ext.toy_pointing_fill(self.xpointing_cpu, nypix_global, nxpix_global, scan_speed, total_drift)
+        self.xpointing_gpu = torch.from_numpy(self.xpointing_cpu).to(_device()) if torch.cuda.is_available() else None
+
+    def __str__(self):
+        shape = (self.ndet, self.nt) if self.ndet is not None else (self.nt,)
+        return (f"ToyPointing(tod_shape={shape}, nypix_global={self.nypix_global}, nxpix_global={self.nxpix_global},"
+                f" scan_speed={self.scan_speed}, total_drift={self.total_drift})")
+
+    @staticmethod
+    def make_random(nsamp_max, noisy=True, rng=None):
+        """Random toy instance with at most nsamp_max samples (gpu_mm.py:1364-1385)."""
+        assert nsamp_max >= 64 * 1024
+        rng = np.random.default_rng() if rng is None else rng
+        if rng.integers(0, 2):
+            ndet = None
+            nt = 32 * int(rng.integers(nsamp_max // 64, nsamp_max // 32))
+        else:
+            r = int(rng.integers(0, 6))
+            s = int((nsamp_max / 32.) ** 0.5)
+            ndet = 2 ** r * int(rng.integers(s // 2, s + 1))
+            nt = 2 ** (5 - r) * int(rng.integers(s // 2, s + 1))
+        npix_max = min(nsamp_max // 128, 16384)
+        nypix_global = int(rng.integers(npix_max / 8, npix_max // 2 + 1))
+        nxpix_global = int(rng.integers(nypix_global + 2, npix_max + 1))
+        scan_speed = float(rng.uniform(0.1, 0.5))
+        total_drift = float(rng.uniform(0.1 * (nxpix_global - nypix_global), (nxpix_global - nypix_global) - 2))
+        return ToyPointing(ndet, nt, nypix_global, nxpix_global, scan_speed, total_drift, noisy=noisy)
+
+
+class PointingPlanTester:
+    """
+    Brute-force plan for tests: iypix/ixpix (the (iy0, ix0) pixel of every sample, computed on
+    the GPU), nmt_cumsum (per planner block) and sorted_mt (46-bit entries without flags,
+    sorted as integers). Reference: gpu_mm/gpu_mm.py:1388-1415; src_lib/PointingPlanTester.cu.
+    """
+
+    def __init__(self, preplan, xpointing_gpu):
+        assert isinstance(preplan, PointingPrePlan)
+        tmp = torch.empty(_ext.plan_tester_tmp_nbytes(preplan), dtype=torch.uint8, device=_device())
+        self.iypix, self.ixpix, self.nmt_cumsum, self.sorted_mt = _ext.plan_tester_run(preplan, xpointing_gpu, tmp, _stream())
+        self.nsamp = preplan.nsamp
+        self.nypix_global = preplan.nypix_global
+        self.nxpix_global = preplan.nxpix_global
+        self.plan_nmt = preplan.plan_nmt
+        self.ncl_per_threadblock = preplan.ncl_per_threadblock
+        self.planner_nblocks = preplan.planner_nblocks
+
+
+def test_plan_iterator(plan_mt, nmt_per_block, warps_per_threadblock):
+    """Walks the planned kernels' work list for a cell-sorted plan_mt and checks coverage."""
+    _ext.test_plan_iterator(np.ascontiguousarray(plan_mt, dtype=np.uint64), int(nmt_per_block), int(warps_per_threadblock))
+
+
+def make_random_plan_mt(ncells, min_nmt_per_cell, max_nmt_per_cell, seed=0):
+    """Random cell-sorted plan (uint64 entries) with junk in the bits above 46."""
+    return _ext.make_random_plan_mt(int(ncells), int(min_nmt_per_cell), int(max_nmt_per_cell), int(seed))

@@ -1,0 +1,222 @@
+/*
+ * gpu_mm_amd: planned pointing operators (map2tod / tod2map) for AMD MI355X (gfx950).
+ *
+ * C ABI of libgpu_mm_amd.so. Plain pointers and sizes only; every array is float32 unless
+ * stated. "gpu" pointers are device memory, "cpu" pointers are host memory. Streams are
+ * hipStream_t passed as void* (NULL = the default stream).
+ *
+ * Every function that can fail returns 0 on success and a nonzero status otherwise; the
+ * message is then available from gmm_last_error() (thread-local). Messages for pointing
+ * errors follow the reference's wording (src_lib/check_arguments.cu:159-175).
+ *
+ * Array conventions (reference gpu_mm/gpu_mm.py:52-84, 117-124):
+ *   tod        float[nsamp]            (a (ndet,nt) array flattened)
+ *   xpointing  float[3][nsamp]         rows {y, x, alpha}
+ *   local map  float[3*npix]           element (p,iy,ix) of cell (cy,cx) is at
+ *              cell_offsets[cy,cx] + p*polstride + iy*ystride + ix
+ *
+ * Each declaration below names the reference interface it replaces (file:line in
+ * kmsmith137/gpu_mm).
+ */
+
+#ifndef GPU_MM_AMD_H
+#define GPU_MM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GMM_PREPLAN_SIZE 1024
+
+/* Error bits reported by the operators (include/gpu_mm_internals.hpp:26-29). */
+#define GMM_ERR_BAD_YPIX 0x1u
+#define GMM_ERR_BAD_XPIX 0x2u
+#define GMM_ERR_INCONSISTENT_NMT 0x4u
+#define GMM_ERR_NOT_IN_PIXELIZATION 0x8u
+
+typedef struct gmm_lpix gmm_lpix;
+typedef struct gmm_preplan gmm_preplan;
+typedef struct gmm_plan gmm_plan;
+
+/* Thread-local message for the last failed call on this thread. */
+const char *gmm_last_error(void);
+
+/* Library version (major*10000 + minor*100 + patch). */
+int gmm_version(void);
+
+/* ---------------------------------------------------------------------------------------
+ * LocalPixelization
+ * Replaces LocalPixelization (include/gpu_mm.hpp:17-58; src_lib/LocalPixelization.cu:33-114;
+ * bound at src_pybind11/gpu_mm_pybind11.cu:77-99). The handle keeps the two caller-owned
+ * cell_offsets pointers (int64[nycells][nxcells], one host copy, one device copy); the caller
+ * keeps both alive and identical.
+ * --------------------------------------------------------------------------------------- */
+
+typedef struct {
+    int64_t nypix_global, nxpix_global;
+    int32_t periodic_xcoord;
+    int64_t ystride, polstride;
+    int64_t nycells, nxcells;
+    int64_t npix; /* local pixels, without the factor 3 for T/Q/U */
+} gmm_lpix_info;
+
+/* LocalPixelization.cu:33-71 (constructor and its stride/offset checks). */
+int gmm_lpix_create(gmm_lpix **out, int64_t nypix_global, int64_t nxpix_global,
+                    const int64_t *cell_offsets_cpu, const int64_t *cell_offsets_gpu,
+                    int64_t nycells, int64_t nxcells, int64_t ystride, int64_t polstride,
+                    int periodic_xcoord);
+void gmm_lpix_destroy(gmm_lpix *lpix);
+int gmm_lpix_get_info(const gmm_lpix *lpix, gmm_lpix_info *out);
+/* gpu_mm_pybind11.cu:94 (npix is read/write). */
+int gmm_lpix_set_npix(gmm_lpix *lpix, int64_t npix);
+/* LocalPixelization.cu:104-114. */
+int gmm_lpix_copy_gpu_offsets_to_cpu(gmm_lpix *lpix);
+int gmm_lpix_copy_cpu_offsets_to_gpu(gmm_lpix *lpix);
+
+/* ---------------------------------------------------------------------------------------
+ * PointingPrePlan
+ * Replaces PointingPrePlan (include/gpu_mm.hpp:61-108; src_lib/PointingPrePlan.cu:129-280;
+ * bound at gpu_mm_pybind11.cu:102-130). nmt_gpu and err_gpu are caller-owned device arrays of
+ * GMM_PREPLAN_SIZE uint32; the preplan keeps nmt_gpu (it holds nmt_cumsum) and uses err_gpu
+ * only during construction. The xpointing array is not retained. Blocks on the stream once.
+ * --------------------------------------------------------------------------------------- */
+
+typedef struct {
+    int64_t nsamp, nypix_global, nxpix_global;
+    int32_t periodic_xcoord;
+    int64_t plan_nbytes;                 /* minimum size of the plan's 'buf' */
+    int64_t plan_constructor_tmp_nbytes; /* minimum size of the plan's 'tmp_buf' */
+    double overhead;                     /* 32*nmt/nsamp - 1 */
+    int64_t ncl_per_threadblock, planner_nblocks;
+    int64_t nmt_per_threadblock, pointing_nblocks; /* reference launch formulas, informational */
+    int64_t plan_nmt;
+    int64_t cub_nbytes;                  /* bytes of radix-sort scratch inside tmp_buf */
+    int64_t max_nchunks;                 /* upper bound on operator work units */
+} gmm_preplan_info;
+
+int gmm_preplan_create(gmm_preplan **out, const float *xpointing_gpu, int64_t nsamp,
+                       int64_t nypix_global, int64_t nxpix_global,
+                       uint32_t *nmt_gpu, uint32_t *err_gpu,
+                       int periodic_xcoord, int debug, void *stream);
+void gmm_preplan_destroy(gmm_preplan *pp);
+int gmm_preplan_get_info(const gmm_preplan *pp, gmm_preplan_info *out);
+/* PointingPrePlan.cu:254-259: copies nmt_cumsum[0:planner_nblocks] to host memory. */
+int gmm_preplan_get_nmt_cumsum(const gmm_preplan *pp, uint32_t *out_cpu, int64_t n);
+
+/* ---------------------------------------------------------------------------------------
+ * PointingPlan
+ * Replaces PointingPlan (include/gpu_mm.hpp:111-152; src_lib/PointingPlan.cu:237-362;
+ * bound at gpu_mm_pybind11.cu:133-153). 'buf' (>= plan_nbytes, 128-byte aligned) is kept by
+ * the plan and must not be shared; 'tmp_buf' (>= plan_constructor_tmp_nbytes) is used only
+ * during construction. Layout of buf: uint64 plan_mt[nmt] at offset 0, then uint32 err[...]
+ * at align128(8*nmt), then the operator work list. Blocks on the stream once.
+ * --------------------------------------------------------------------------------------- */
+
+typedef struct {
+    int64_t nsamp, nypix_global, nxpix_global;
+    int32_t periodic_xcoord;
+    int64_t plan_nmt;
+    int64_t nchunks;       /* operator work units (blocks per map2tod/tod2map launch) */
+    int64_t nmlines;       /* number of 32-sample lines that touch more than one cell */
+} gmm_plan_info;
+
+int gmm_plan_create(gmm_plan **out, const gmm_preplan *pp, const float *xpointing_gpu,
+                    int64_t nsamp, void *buf_gpu, int64_t buf_nbytes,
+                    void *tmp_buf_gpu, int64_t tmp_nbytes, int debug, void *stream);
+void gmm_plan_destroy(gmm_plan *plan);
+int gmm_plan_get_info(const gmm_plan *plan, gmm_plan_info *out);
+/* PointingPlan.cu:319-327: copies plan_mt (sorted by cell) to 'out' (host or device). */
+int gmm_plan_get_plan_mt(const gmm_plan *plan, uint64_t *out, int64_t n, int out_on_gpu,
+                         void *stream);
+/* PointingPlan.cu:331-335: synchronous check of the plan's error flags. */
+int gmm_plan_check_errflags(const gmm_plan *plan, const char *where);
+
+/* ---------------------------------------------------------------------------------------
+ * Operators. map2tod: tod (=, or += when accum) A * map. tod2map: map += A^T * tod.
+ * 'debug' makes the call check error flags synchronously (as the reference always does);
+ * with check_errors == 0 and debug == 0 the call is asynchronous and errors are only
+ * visible through gmm_plan_check_errflags().
+ * --------------------------------------------------------------------------------------- */
+
+/* launch_planned_map2tod, include/gpu_mm.hpp:155-165; src_lib/map2tod.cu:194-272. */
+int gmm_planned_map2tod(float *tod_gpu, int64_t tod_nsamp,
+                        const float *local_map_gpu, int64_t local_map_size,
+                        const float *xpointing_gpu, int64_t xpointing_nsamp,
+                        const gmm_lpix *lpix, const gmm_plan *plan,
+                        int partial_pixelization, int debug, int accum, int check_errors,
+                        void *stream);
+
+/* launch_planned_tod2map, include/gpu_mm.hpp:168-177; src_lib/tod2map.cu:157-226. */
+int gmm_planned_tod2map(float *local_map_gpu, int64_t local_map_size,
+                        const float *tod_gpu, int64_t tod_nsamp,
+                        const float *xpointing_gpu, int64_t xpointing_nsamp,
+                        const gmm_lpix *lpix, const gmm_plan *plan,
+                        int partial_pixelization, int debug, int check_errors, void *stream);
+
+/* launch_unplanned_map2tod, include/gpu_mm.hpp:204-213; src_lib/map2tod_unplanned.cu:74-127.
+ * errflags_gpu: caller-supplied uint32[nerrflags]; the launch uses at most nerrflags blocks. */
+int gmm_unplanned_map2tod(float *tod_gpu, int64_t tod_nsamp,
+                          const float *local_map_gpu, int64_t local_map_size,
+                          const float *xpointing_gpu, int64_t xpointing_nsamp,
+                          const gmm_lpix *lpix, uint32_t *errflags_gpu, int64_t nerrflags,
+                          int partial_pixelization, int accum, int check_errors, void *stream);
+
+/* launch_unplanned_tod2map, include/gpu_mm.hpp:216-224; src_lib/tod2map_unplanned.cu:68-121. */
+int gmm_unplanned_tod2map(float *local_map_gpu, int64_t local_map_size,
+                          const float *tod_gpu, int64_t tod_nsamp,
+                          const float *xpointing_gpu, int64_t xpointing_nsamp,
+                          const gmm_lpix *lpix, uint32_t *errflags_gpu, int64_t nerrflags,
+                          int partial_pixelization, int check_errors, void *stream);
+
+/* reference_map2tod, include/gpu_mm.hpp:227-236; src_lib/map2tod_reference.cu:16-53.
+ * Host arrays; single-threaded. */
+int gmm_reference_map2tod(float *tod_cpu, int64_t tod_nsamp,
+                          const float *local_map_cpu, int64_t local_map_size,
+                          const float *xpointing_cpu, int64_t xpointing_nsamp,
+                          const gmm_lpix *lpix, int partial_pixelization, int accum);
+
+/* reference_tod2map, include/gpu_mm.hpp:239-247; src_lib/tod2map_reference.cu:14-46. */
+int gmm_reference_tod2map(float *local_map_cpu, int64_t local_map_size,
+                          const float *tod_cpu, int64_t tod_nsamp,
+                          const float *xpointing_cpu, int64_t xpointing_nsamp,
+                          const gmm_lpix *lpix, int partial_pixelization);
+
+/* ---------------------------------------------------------------------------------------
+ * Test aids.
+ * --------------------------------------------------------------------------------------- */
+
+/* ToyPointing, include/gpu_mm.hpp:314-350; src_lib/ToyPointing.cu:41-109. Fills a host
+ * xpointing[3][nsamp] array with the 45-degree bouncing toy scan; returns drift_speed. */
+int gmm_toy_pointing_fill(float *xpointing_cpu, int64_t nsamp, int64_t nypix_global,
+                          int64_t nxpix_global, double scan_speed, double total_drift,
+                          double *drift_speed_out);
+
+/* PointingPlanTester, include/gpu_mm.hpp:388-430; src_lib/PointingPlanTester.cu:69-178.
+ * Quantizes the pointing on the GPU (iypix, ixpix = (iy0, ix0) of each sample) and builds the
+ * brute-force plan on the host: nmt_cumsum[planner_nblocks] and sorted_mt[plan_nmt]
+ * (46-bit entries, no flags, sorted as integers). */
+int64_t gmm_plan_tester_tmp_nbytes(const gmm_preplan *pp);
+int gmm_plan_tester_run(const gmm_preplan *pp, const float *xpointing_gpu, int64_t nsamp,
+                        void *tmp_gpu, int64_t tmp_nbytes,
+                        int32_t *iypix_cpu, int32_t *ixpix_cpu,
+                        uint32_t *nmt_cumsum_cpu, uint64_t *sorted_mt_cpu, void *stream);
+
+/* test_plan_iterator, include/plan_iterator.hpp:14; src_lib/test_plan_iterator.cu:120-166.
+ * Builds the operator work list for an arbitrary cell-sorted plan_mt (chunks of at most
+ * nmt_per_block entries), walks it on the GPU with 'waves_per_block' waves per block and
+ * checks that every entry is visited exactly once by a block working on its cell. */
+int gmm_test_plan_iterator(const uint64_t *plan_mt_cpu, int64_t nmt, int64_t nmt_per_block,
+                           int waves_per_block);
+
+/* make_random_plan_mt, include/plan_iterator.hpp:15; src_lib/test_plan_iterator.cu:170-209.
+ * Seeded. Call with out_cpu == NULL to get the entry count in *nmt_out first. */
+int gmm_make_random_plan_mt(int64_t ncells, int64_t min_nmt_per_cell, int64_t max_nmt_per_cell,
+                            uint64_t seed, uint64_t *out_cpu, int64_t capacity, int64_t *nmt_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPU_MM_AMD_H */

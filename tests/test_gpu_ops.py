@@ -1,0 +1,278 @@
+"""GPU parity tests: the HIP planner and operators against the oracle on the same seeded inputs.
+
+Bars: plans bit-exact (all 48 bits, canonical order) and nmt_cumsum equal; FP32 operators
+within 1e-6 L1-relative of the FP32 oracle (the reference's own metric and threshold,
+gpu_mm/tests.py:220-224, 277, 326) and within 1e-5 of the FP64 oracle; adjoint identity
+<A m, d> = <m, A^T d> to 1e-6 relative.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from gpu_mm_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+F32 = np.float32
+
+
+def l1rel(a, b):
+    a = np.asarray(a, dtype=np.float64).reshape(-1)
+    b = np.asarray(b, dtype=np.float64).reshape(-1)
+    return float(np.sum(np.abs(a - b)) / (np.sum(np.abs(a)) + np.sum(np.abs(b))))
+
+
+def to_gpu(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def to_cpu(t):
+    return t.detach().to("cpu").numpy()
+
+
+def make_instance(kind, seed):
+    """Returns (xpointing (3,ndet,nt) float32, ny, nx, periodic)."""
+    rng = np.random.default_rng(seed)
+    if kind == "cfg0":
+        d = synth.make_cfg0(seed=0)
+        return d["xpointing"], d["ny"], d["nx"], False
+    if kind == "fast_periodic":
+        # Fast scan with wrap-around: many multi-cell lines, cells on both ends of the map.
+        ndet, nt, ny, nx = 8, 32 * 512, 300, 700
+        t = np.arange(nt)
+        xp = np.empty((3, ndet, nt), F32)
+        for d in range(ndet):
+            xp[0, d] = 150 + 140 * np.sin(t / 700.0 + d) + rng.uniform(-1, 1)
+            xp[1, d] = (t * (1.7 + 0.3 * d)) % (2 * nx) - nx / 2
+            xp[2, d] = rng.uniform(0, np.pi) + 0.01 * t
+        return xp, ny, nx, True
+    if kind == "toy":
+        from gpu_mm_amd import ToyPointing
+        tp = ToyPointing.make_random(1 << 20, noisy=False, rng=rng)
+        xp = tp.xpointing_cpu.reshape(3, -1, tp.xpointing_cpu.shape[-1])
+        return xp, tp.nypix_global, tp.nxpix_global, False
+    raise ValueError(kind)
+
+
+INSTANCES = [("cfg0", 0), ("fast_periodic", 1), ("toy", 2), ("toy", 3)]
+
+
+@pytest.fixture(scope="module", params=INSTANCES, ids=lambda p: f"{p[0]}-{p[1]}")
+def inst(request, gmm, dev):
+    kind, seed = request.param
+    xp, ny, nx, periodic = make_instance(kind, seed)
+    print(f"\n[instance {kind} seed={seed}] xpointing {xp.shape} ny={ny} nx={nx} periodic={periodic}", flush=True)
+    xg = to_gpu(xp, dev)
+    pre = gmm.PointingPrePlan(xg, ny, nx, periodic_xcoord=periodic)
+    plan = gmm.PointingPlan(pre, xg)
+    lp = gmm.LocalPixelization.make_rectangle(ny, nx, periodic_xcoord=periodic)
+    return dict(xp=xp, xg=xg, ny=ny, nx=nx, periodic=periodic, pre=pre, plan=plan, lp=lp, seed=seed)
+
+
+def opix(orc, inst):
+    return orc.Pix.rectangle(inst["ny"], inst["nx"], inst["periodic"])
+
+
+def test_preplan_matches_oracle(inst, orc):
+    cum, nmt, err = orc.plan_count(inst["xp"], inst["ny"], inst["nx"], inst["periodic"])
+    assert err == 0
+    pre = inst["pre"]
+    assert pre.plan_nmt == nmt
+    assert np.array_equal(pre.get_nmt_cumsum(), cum)
+    per, nb = orc.preplan_params(pre.nsamp)
+    assert (pre.ncl_per_threadblock, pre.planner_nblocks) == (per, nb)
+    assert (pre.nmt_per_threadblock, pre.pointing_nblocks) == orc.pointing_params(nmt)
+    assert pre.overhead == pytest.approx(32 * nmt / pre.nsamp - 1)
+
+
+def test_plan_bit_exact(inst, orc):
+    mt = inst["plan"].get_plan_mt()
+    ref, err = orc.plan_build(inst["xp"], inst["ny"], inst["nx"], inst["periodic"])
+    assert err == 0
+    assert mt.dtype == np.uint64 and mt.shape == ref.shape
+    assert np.array_equal(mt, ref)
+
+
+def test_plan_matches_plan_tester(inst, gmm):
+    """The reference's own criterion (gpu_mm/tests.py:227-254)."""
+    tester = gmm.PointingPlanTester(inst["pre"], inst["xg"])
+    assert np.array_equal(inst["pre"].get_nmt_cumsum(), tester.nmt_cumsum)
+    mt = inst["plan"].get_plan_mt()
+    low20 = mt & np.uint64((1 << 20) - 1)
+    assert np.all(low20[1:] >= low20[:-1])
+    assert np.array_equal(np.sort(mt & np.uint64((1 << 46) - 1)), tester.sorted_mt)
+
+
+def test_plan_iterator_on_plan(inst, gmm):
+    mt = inst["plan"].get_plan_mt()
+    for nper, waves in ((32, 4), (1000, 8), (4096, 16)):
+        gmm.test_plan_iterator(mt, nper, waves)
+
+
+@pytest.mark.parametrize("accum", [False, True])
+def test_map2tod(inst, gmm, orc, dev, accum):
+    rng = np.random.default_rng(100 + inst["seed"] + 7 * accum)
+    lp, plan = inst["lp"], inst["plan"]
+    m = rng.standard_normal(3 * lp.npix).astype(F32)
+    tod0 = rng.standard_normal(inst["xp"].shape[1:]).astype(F32)
+    pix = opix(orc, inst)
+    ref32, err = orc.map2tod(m, inst["xp"], pix, tod=tod0, accum=accum)
+    assert err == 0
+    ref64, _ = orc.map2tod(m.astype(np.float64), inst["xp"], pix, tod=tod0, accum=accum, f64=True)
+    mg = gmm.LocalMap(lp, to_gpu(m, dev))
+    for p in (plan, None):
+        t = to_gpu(tod0 if accum else rng.standard_normal(tod0.shape).astype(F32), dev)
+        gmm.map2tod(t, mg, inst["xg"], p, accum=accum, debug=True)
+        got = to_cpu(t)
+        e32, e64 = l1rel(got, ref32), l1rel(got, ref64)
+        print(f"  map2tod plan={'planned' if p is not None else 'unplanned'} accum={accum}: "
+              f"l1rel vs f32 oracle {e32:.2e}, vs f64 {e64:.2e}")
+        assert e32 < 1e-6
+        assert e64 < 1e-5
+
+
+def test_tod2map(inst, gmm, orc, dev):
+    rng = np.random.default_rng(200 + inst["seed"])
+    lp, plan = inst["lp"], inst["plan"]
+    tod = rng.standard_normal(inst["xp"].shape[1:]).astype(F32)
+    m0 = rng.standard_normal(3 * lp.npix).astype(F32)
+    pix = opix(orc, inst)
+    ref32, err = orc.tod2map(tod, inst["xp"], pix, lmap=m0)
+    assert err == 0
+    ref64, _ = orc.tod2map(tod.astype(np.float64), inst["xp"], pix, lmap=m0.astype(np.float64), f64=True)
+    tg = to_gpu(tod, dev)
+    for p in (plan, None):
+        mg = gmm.LocalMap(lp, to_gpu(m0, dev))
+        gmm.tod2map(mg, tg, inst["xg"], p, debug=True)
+        got = to_cpu(mg.arr)
+        e32, e64 = l1rel(got, ref32), l1rel(got, ref64)
+        print(f"  tod2map plan={'planned' if p is not None else 'unplanned'}: "
+              f"l1rel vs f32 oracle {e32:.2e}, vs f64 {e64:.2e}")
+        assert e32 < 1e-6
+        assert e64 < 1e-5
+
+
+def test_adjoint(inst, gmm, dev):
+    rng = np.random.default_rng(300 + inst["seed"])
+    lp, plan = inst["lp"], inst["plan"]
+    m = rng.standard_normal(3 * lp.npix).astype(F32)
+    d = rng.standard_normal(inst["xp"].shape[1:]).astype(F32)
+    am = torch.zeros(d.shape, dtype=torch.float32, device=dev)
+    gmm.map2tod(am, gmm.LocalMap(lp, to_gpu(m, dev)), inst["xg"], plan)
+    atd = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
+    gmm.tod2map(atd, to_gpu(d, dev), inst["xg"], plan)
+    lhs = float(np.dot(to_cpu(am).astype(np.float64).ravel(), d.astype(np.float64).ravel()))
+    rhs = float(np.dot(m.astype(np.float64), to_cpu(atd.arr).astype(np.float64)))
+    scale = np.linalg.norm(to_cpu(am)) * np.linalg.norm(d)
+    assert abs(lhs - rhs) < 1e-6 * scale
+
+
+# ------------------------------------------------------------------------------------------
+# Edge cases.
+
+def test_partial_pixelization_gpu(gmm, orc, dev):
+    rng = np.random.default_rng(7)
+    ny, nx = 256, 320
+    cells = [(cy, cx) for cy in range(4) for cx in range(5) if (cy + cx) % 3 != 0]
+    lp = gmm.LocalPixelization.make_cells(ny, nx, cells, periodic_xcoord=False)
+    pix = orc.Pix(ny, nx, lp.cell_offsets_cpu, 64, 4096, False)
+    n = 32 * 2048
+    t = np.arange(n)
+    xp = np.stack([128 + 120 * np.sin(t / 900.0), 160 + 150 * np.sin(t / 1300.0 + 1), 0.002 * t]).astype(F32)
+    xg = to_gpu(xp, dev)
+    pre = gmm.PointingPrePlan(xg, ny, nx, periodic_xcoord=False)
+    plan = gmm.PointingPlan(pre, xg)
+    m = rng.standard_normal(3 * lp.npix).astype(F32)
+    mg = gmm.LocalMap(lp, to_gpu(m, dev))
+    tg = torch.zeros(n, dtype=torch.float32, device=dev)
+    for p in (plan, None):
+        with pytest.raises(RuntimeError, match="not in LocalPixelization"):
+            gmm.map2tod(tg, mg, xg, p)
+        with pytest.raises(RuntimeError, match="not in LocalPixelization"):
+            gmm.tod2map(gmm.LocalMap(lp, torch.zeros_like(mg.arr)), tg, xg, p)
+    ref, err = orc.map2tod(m, xp, pix, partial=True)
+    assert err == 0
+    d = rng.standard_normal(n).astype(F32)
+    mref, _ = orc.tod2map(d, xp, pix, partial=True)
+    for p in (plan, None):
+        gmm.map2tod(tg, mg, xg, p, partial_pixelization=True)
+        assert l1rel(to_cpu(tg), ref) < 1e-6
+        out = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
+        gmm.tod2map(out, to_gpu(d, dev), xg, p, partial_pixelization=True)
+        assert l1rel(to_cpu(out.arr), mref) < 1e-6
+
+
+def test_bad_pointing_raises(gmm, dev):
+    ny, nx = 128, 128
+    n = 32 * 64
+    xp = np.zeros((3, n), F32)
+    xp[0] = 10.0
+    xp[1] = np.linspace(0, 120, n)
+    xp[0, 77] = 200.0
+    xg = to_gpu(xp, dev)
+    with pytest.raises(RuntimeError, match="y-value is outside range"):
+        gmm.PointingPrePlan(xg, ny, nx, periodic_xcoord=False)
+    xp[0, 77] = 10.0
+    xp[1, 99] = 127.5
+    with pytest.raises(RuntimeError, match="periodic_xcoord"):
+        gmm.PointingPrePlan(to_gpu(xp, dev), ny, nx, periodic_xcoord=False)
+    # Valid plan, then pointing goes bad: the operators report it.
+    xp[1, 99] = 50.0
+    xg = to_gpu(xp, dev)
+    pre = gmm.PointingPrePlan(xg, ny, nx, periodic_xcoord=False)
+    plan = gmm.PointingPlan(pre, xg)
+    lp = gmm.LocalPixelization.make_rectangle(ny, nx, periodic_xcoord=False)
+    mg = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
+    tg = torch.zeros(n, dtype=torch.float32, device=dev)
+    xg[0, 5] = -3.0
+    for p in (plan, None):
+        with pytest.raises(RuntimeError, match="y-value"):
+            gmm.map2tod(tg, mg, xg, p)
+    # The flags were cleared by the failed check: a good call succeeds.
+    xg[0, 5] = 10.0
+    gmm.map2tod(tg, mg, xg, plan)
+
+
+def test_plan_from_other_pointing_raises(gmm, dev):
+    ny, nx = 256, 256
+    n = 32 * 1024
+    t = np.arange(n)
+    a = np.stack([20 + 0.006 * t, 128 + 100 * np.sin(t / 300.0), 0 * t]).astype(F32)
+    b = np.stack([20 + 0.006 * t, 128 + 100 * np.sin(t / 30.0), 0 * t]).astype(F32)
+    pre = gmm.PointingPrePlan(to_gpu(a, dev), ny, nx, periodic_xcoord=False)
+    with pytest.raises(RuntimeError, match="inconsistent value of nmt"):
+        gmm.PointingPlan(pre, to_gpu(b, dev))
+
+
+def test_small_buffers_raise(gmm, dev):
+    xp = to_gpu(synth.make_cfg0(seed=0, ndet=2, nt=4096)["xpointing"], dev)
+    pre = gmm.PointingPrePlan(xp, 256, 512, periodic_xcoord=False)
+    small = torch.empty(pre.plan_nbytes - 8, dtype=torch.uint8, device=dev)
+    with pytest.raises(RuntimeError, match="'buf'"):
+        gmm.PointingPlan(pre, xp, buf=small)
+    small = torch.empty(pre.plan_constructor_tmp_nbytes - 8, dtype=torch.uint8, device=dev)
+    with pytest.raises(RuntimeError, match="'tmp_buf'"):
+        gmm.PointingPlan(pre, xp, tmp_buf=small)
+
+
+@pytest.mark.parametrize("ncells,lo,hi,nper,waves", [(100, 1, 1000, 37, 4), (700, 1, 50, 512, 8),
+                                                     (1, 1, 16384, 4096, 16), (1000, 5, 5, 1, 4)])
+def test_plan_iterator_random(gmm, ncells, lo, hi, nper, waves):
+    mt = gmm.make_random_plan_mt(ncells, lo, hi, seed=ncells + nper)
+    gmm.test_plan_iterator(mt, nper, waves)
+
+
+def test_shared_plan_buffers(gmm, dev):
+    """tmp_buf may be shared by several plans; buf may not (gpu_mm.py:1227-1232)."""
+    d0 = synth.make_cfg0(seed=0, ndet=4, nt=8192)
+    d1 = synth.make_cfg0(seed=1, ndet=4, nt=8192)
+    x0, x1 = to_gpu(d0["xpointing"], dev), to_gpu(d1["xpointing"], dev)
+    p0 = gmm.PointingPrePlan(x0, 256, 512, periodic_xcoord=False)
+    p1 = gmm.PointingPrePlan(x1, 256, 512, periodic_xcoord=False)
+    tmp = torch.empty(max(p0.plan_constructor_tmp_nbytes, p1.plan_constructor_tmp_nbytes), dtype=torch.uint8, device=dev)
+    a = gmm.PointingPlan(p0, x0, tmp_buf=tmp)
+    b = gmm.PointingPlan(p1, x1, tmp_buf=tmp)
+    a2 = gmm.PointingPlan(p0, x0)
+    assert np.array_equal(a.get_plan_mt(), a2.get_plan_mt())
+    assert b.plan_nmt == p1.plan_nmt
