@@ -4,7 +4,8 @@ Products (all in-tree, next to this file):
   libgpu_mm_amd.so            HIP kernels for gfx950 + the C ABI (include/gpu_mm_amd.h)
   _gpu_mm_amd_py<ext>.so      pybind11 module over the C ABI (arrays in via DLPack)
 
-Run as ``python -m gpu_mm_amd._build`` or through ``__graft_entry__.build()``.
+Run as ``python gpu_mm_amd/_build.py`` or through ``__graft_entry__.build()`` (the package
+itself cannot be imported before the extension exists).
 """
 
 import os
@@ -63,7 +64,7 @@ def build_lib(verbose=False):
     def compile_one(item):
         src, extra = item
         obj = os.path.join(BUILD_DIR, src + ".o")
-        cmd = [HIPCC] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC] + HIP_FLAGS + extra + [f"-I{os.path.join(REPO_DIR, 'include')}", "-c", os.path.join(CSRC, src), "-o", obj]
         out = _run(cmd)
         if verbose and out.strip():
             print(out)

@@ -21,7 +21,7 @@ def ext():
             _mod = importlib.import_module("gpu_mm_amd._gpu_mm_amd_py")
         except ImportError as e:
             raise ImportError(
-                "gpu_mm_amd: native extension not built (run `python -m gpu_mm_amd._build` "
+                "gpu_mm_amd: native extension not built (run `python gpu_mm_amd/_build.py` "
                 f"or __graft_entry__.build()): {e}") from e
     return _mod
 

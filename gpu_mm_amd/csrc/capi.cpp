@@ -2,7 +2,7 @@
 // lifetime, and the host side of plan construction and operator launches.
 
 #include "gmm_internal.hpp"
-#include "../../include/gpu_mm_amd.h"
+#include "gpu_mm_amd.h"
 
 #include <algorithm>
 #include <cmath>
@@ -338,7 +338,8 @@ int gmm_preplan_create(gmm_preplan **out, const float *xpointing_gpu, int64_t ns
         pp.off_err = align128(8 * nmt);
         pp.off_chunks = pp.off_err + align128(4 * pp.n_err);
         pp.off_mlines = pp.off_chunks + align128(int64_t(sizeof(Chunk)) * pp.max_nchunks);
-        pp.plan_nbytes = pp.off_mlines + align128(4 * std::max<int64_t>(pp.max_nmlines, 1));
+        pp.off_absmax = pp.off_mlines + align128(4 * std::max<int64_t>(pp.max_nmlines, 1));
+        pp.plan_nbytes = pp.off_absmax + align128(4 * tod2map_scratch_words());
 
         pp.sort_nbytes = plan_scratch_nbytes(pp) + 128;
         pp.tmp_nbytes = 2 * align128(8 * nmt) + pp.sort_nbytes;
@@ -423,6 +424,7 @@ int gmm_plan_create(gmm_plan **out, const gmm_preplan *pp, const float *xpointin
         plan->err = reinterpret_cast<uint32_t *>(plan->buf + pp->off_err);
         plan->chunks = reinterpret_cast<Chunk *>(plan->buf + pp->off_chunks);
         plan->mlines = reinterpret_cast<uint32_t *>(plan->buf + pp->off_mlines);
+        plan->absmax = reinterpret_cast<uint32_t *>(plan->buf + pp->off_absmax);
 
         try {
             uint8_t *tmp = reinterpret_cast<uint8_t *>(tmp_buf_gpu);

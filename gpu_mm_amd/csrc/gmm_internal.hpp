@@ -44,7 +44,7 @@ struct gmm_preplan
     int64_t plan_nbytes = 0, tmp_nbytes = 0, sort_nbytes = 0;
 
     // Offsets inside the plan buffer.
-    int64_t off_err = 0, off_chunks = 0, off_mlines = 0, n_err = 0;
+    int64_t off_err = 0, off_chunks = 0, off_mlines = 0, off_absmax = 0, n_err = 0;
 };
 
 struct gmm_plan
@@ -55,6 +55,7 @@ struct gmm_plan
     uint32_t *err = nullptr;        // per-block error words (n_err of them)
     gmm::Chunk *chunks = nullptr;   // operator work list
     uint32_t *mlines = nullptr;     // lines that touch more than one cell
+    uint32_t *absmax = nullptr;     // tod2map scratch: per-block max|tod| partials
     int64_t nchunks = 0;
     int64_t nmlines = 0;
 };
@@ -149,6 +150,8 @@ void launch_planned_map2tod(float *tod, const float *lmap, const float *xp, cons
                             const gmm_plan &plan, bool accum, hipStream_t stream);
 void launch_planned_tod2map(float *lmap, const float *tod, const float *xp, const OpArgs &a,
                             const gmm_plan &plan, hipStream_t stream);
+// Words of tod2map scratch kept in the plan buffer.
+int64_t tod2map_scratch_words();
 
 int64_t unplanned_nblocks(int64_t nsamp, int64_t max_blocks);
 void launch_unplanned_map2tod(float *tod, const float *lmap, const float *xp, const OpArgs &a,

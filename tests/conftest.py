@@ -13,9 +13,12 @@ def pytest_configure(config):
 
 
 def _ensure_built():
-    from gpu_mm_amd import _build
-    if not (os.path.exists(_build.lib_path()) and os.path.exists(_build.ext_path())):
-        _build.build()
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("gpu_mm_amd_build", os.path.join(REPO, "gpu_mm_amd", "_build.py"))
+    b = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(b)
+    if not (os.path.exists(b.lib_path()) and os.path.exists(b.ext_path())):
+        b.build()
     from oracle import pyoracle
     if not os.path.exists(pyoracle.LIB_PATH):
         pyoracle.build()
