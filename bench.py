@@ -126,6 +126,8 @@ def main():
     ap.add_argument("--nt", type=int, default=262144)
     ap.add_argument("--cpu-ndet", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-ref", action="store_true",
+                    help="also run a plain 1 GiB device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -199,6 +201,14 @@ def main():
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("bytes_per_launch")
+
+    if args.copy_ref:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     t_pre, t_plan = time_plan_construction(torch, gmm, w)
     log(f"[bench] PointingPrePlan {t_pre:.2f} ms, PointingPlan {t_plan:.2f} ms")

@@ -37,6 +37,7 @@ LIB_SOURCES = [
     ("k_planner.hip", []),
     ("k_planned.hip", []),
     ("k_unplanned.hip", []),
+    ("k_tod16.hip", []),
 ]
 
 LIB_NAME = "libgpu_mm_amd.so"

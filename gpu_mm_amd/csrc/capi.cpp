@@ -5,7 +5,9 @@
 #include "gpu_mm_amd.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <mutex>
 #include <cstring>
 #include <sstream>
 
@@ -40,6 +42,39 @@ void throw_errflags(uint32_t err, const char *where, uint32_t ignore)
     if (err & ERR_NOT_IN_PIXELIZATION)
         fail(w + ": xpointing (y,x) value is not in LocalPixelization (perhaps pixelization is too small, or you want the 'partial_pixelization' flag");
     fail(w + ": bad errflags?! (should never happen)");
+}
+
+// ------------------------------------------------------------------------------------------
+// Kernel timer: HIP event pairs recorded on the launch stream around each main operator kernel
+// while enabled (a measurement aid; off by default, one relaxed load per launch when off).
+
+namespace {
+std::atomic<bool> g_timing{false};
+std::mutex g_timer_mu;
+std::vector<std::pair<hipEvent_t, hipEvent_t>> g_timer_ev[TK_COUNT];
+size_t g_timer_used[TK_COUNT] = {};
+}  // namespace
+
+void kernel_timer_mark(int which, bool end, hipStream_t stream)
+{
+    if (!g_timing.load(std::memory_order_relaxed))
+        return;
+    std::lock_guard<std::mutex> lk(g_timer_mu);
+    auto &v = g_timer_ev[which];
+    size_t &u = g_timer_used[which];
+    if (!end) {
+        if (u == v.size()) {
+            hipEvent_t a, b;
+            GMM_HIP(hipEventCreate(&a));
+            GMM_HIP(hipEventCreate(&b));
+            v.emplace_back(a, b);
+        }
+        GMM_HIP(hipEventRecord(v[u].first, stream));
+    }
+    else if (u < v.size()) {
+        GMM_HIP(hipEventRecord(v[u].second, stream));
+        u++;
+    }
 }
 
 void check_nsamp(int64_t nsamp, const char *where)
@@ -180,6 +215,37 @@ extern "C" {
 const char *gmm_last_error(void) { return g_last_error.c_str(); }
 
 int gmm_version(void) { return 100; }
+
+int gmm_kernel_timer_enable(int enable)
+{
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(g_timer_mu);
+        for (int k = 0; k < TK_COUNT; k++)
+            g_timer_used[k] = 0;
+        g_timing.store(enable != 0);
+    });
+}
+
+int gmm_kernel_timer_read(int which, double *total_ms, int64_t *count)
+{
+    return guarded([&] {
+        if (which < 0 || which >= TK_COUNT)
+            fail("kernel_timer_read: unknown kernel index");
+        require_ptr(total_ms, "kernel_timer_read", "total_ms");
+        require_ptr(count, "kernel_timer_read", "count");
+        std::lock_guard<std::mutex> lk(g_timer_mu);
+        double t = 0.0;
+        for (size_t i = 0; i < g_timer_used[which]; i++) {
+            GMM_HIP(hipEventSynchronize(g_timer_ev[which][i].second));
+            float ms = 0.0f;
+            GMM_HIP(hipEventElapsedTime(&ms, g_timer_ev[which][i].first, g_timer_ev[which][i].second));
+            t += ms;
+        }
+        *total_ms = t;
+        *count = int64_t(g_timer_used[which]);
+        g_timer_used[which] = 0;
+    });
+}
 
 // ------------------------------------------------------------------------------------------
 
@@ -519,7 +585,7 @@ int gmm_planned_map2tod(float *tod_gpu, int64_t tod_nsamp,
         check_op_common(lpix, plan, tod_nsamp, xpointing_nsamp, local_map_size, where);
         require_ptr(lpix->offsets_gpu, where, "cell_offsets_gpu");
         OpArgs a{lpix, tod_nsamp, partial_pixelization != 0};
-        launch_planned_map2tod(tod_gpu, local_map_gpu, xpointing_gpu, a, *plan, accum != 0, stream);
+        launch_planned_map2tod(tod_gpu, local_map_gpu, xpointing_gpu, a, *plan, accum != 0, TodGains{}, stream);
         if (debug || check_errors)
             check_plan_errors(*plan, where, stream);
     });
@@ -541,9 +607,173 @@ int gmm_planned_tod2map(float *local_map_gpu, int64_t local_map_size,
         check_op_common(lpix, plan, tod_nsamp, xpointing_nsamp, local_map_size, where);
         require_ptr(lpix->offsets_gpu, where, "cell_offsets_gpu");
         OpArgs a{lpix, tod_nsamp, partial_pixelization != 0};
-        launch_planned_tod2map(local_map_gpu, tod_gpu, xpointing_gpu, a, *plan, stream);
+        launch_planned_tod2map(local_map_gpu, tod_gpu, xpointing_gpu, a, *plan, TodGains{}, stream);
         if (debug || check_errors)
             check_plan_errors(*plan, where, stream);
+    });
+}
+
+namespace {
+// check_response(), src_lib/check_arguments.cu:171-180: response is float[2][ndet] (row 0: T
+// response, row 1: polarization response) and ndet must divide nsamp.
+void check_response(const float *response_gpu, int64_t ndet, int64_t nsamp, const char *where)
+{
+    require_ptr(response_gpu, where, "response");
+    if (ndet <= 0)
+        fail(std::string(where) + ": response array must have shape (2, ndet) with ndet > 0");
+    if (nsamp % ndet != 0)
+        fail(std::string(where) + ": response array has ndet=" + std::to_string(ndet)
+             + ", which does not divide nsamp=" + std::to_string(nsamp));
+    if (nsamp / ndet > int64_t(UINT32_MAX))
+        fail(std::string(where) + ": samples per detector must be < 2^32");
+}
+}  // namespace
+
+int gmm_response_map2tod(float *tod_gpu, int64_t tod_nsamp,
+                         const float *local_map_gpu, int64_t local_map_size,
+                         const float *xpointing_gpu, int64_t xpointing_nsamp,
+                         const float *response_gpu, int64_t ndet,
+                         const gmm_lpix *lpix, const gmm_plan *plan,
+                         int partial_pixelization, int debug, int accum, int check_errors,
+                         void *stream_)
+{
+    return guarded([&] {
+        const char *where = "response_map2tod";
+        hipStream_t stream = as_stream(stream_);
+        require_ptr(plan, where, "plan");
+        require_ptr(tod_gpu, where, "tod");
+        require_ptr(local_map_gpu, where, "local_map");
+        require_ptr(xpointing_gpu, where, "xpointing");
+        check_op_common(lpix, plan, tod_nsamp, xpointing_nsamp, local_map_size, where);
+        check_response(response_gpu, ndet, tod_nsamp, where);
+        require_ptr(lpix->offsets_gpu, where, "cell_offsets_gpu");
+        OpArgs a{lpix, tod_nsamp, partial_pixelization != 0};
+        TodGains g;
+        g.response = response_gpu;
+        g.resp_ndet = ndet;
+        launch_planned_map2tod(tod_gpu, local_map_gpu, xpointing_gpu, a, *plan, accum != 0, g, stream);
+        if (debug || check_errors)
+            check_plan_errors(*plan, where, stream);
+    });
+}
+
+int gmm_response_tod2map(float *local_map_gpu, int64_t local_map_size,
+                         const float *tod_gpu, int64_t tod_nsamp,
+                         const float *xpointing_gpu, int64_t xpointing_nsamp,
+                         const float *response_gpu, int64_t ndet,
+                         const gmm_lpix *lpix, const gmm_plan *plan,
+                         int partial_pixelization, int debug, int check_errors, void *stream_)
+{
+    return guarded([&] {
+        const char *where = "response_tod2map";
+        hipStream_t stream = as_stream(stream_);
+        require_ptr(plan, where, "plan");
+        require_ptr(tod_gpu, where, "tod");
+        require_ptr(local_map_gpu, where, "local_map");
+        require_ptr(xpointing_gpu, where, "xpointing");
+        check_op_common(lpix, plan, tod_nsamp, xpointing_nsamp, local_map_size, where);
+        check_response(response_gpu, ndet, tod_nsamp, where);
+        require_ptr(lpix->offsets_gpu, where, "cell_offsets_gpu");
+        OpArgs a{lpix, tod_nsamp, partial_pixelization != 0};
+        TodGains g;
+        g.response = response_gpu;
+        g.resp_ndet = ndet;
+        launch_planned_tod2map(local_map_gpu, tod_gpu, xpointing_gpu, a, *plan, g, stream);
+        if (debug || check_errors)
+            check_plan_errors(*plan, where, stream);
+    });
+}
+
+namespace {
+// Arguments of the 16-bit TOD entry points.
+TodGains q16_gains(int tod_type, const void *tod, const float *scale, int64_t nrows, int64_t nsamp,
+                   const float *response, int64_t resp_ndet, const char *where)
+{
+    if (tod_type != GMM_TOD_F16 && tod_type != GMM_TOD_BF16)
+        fail(std::string(where) + ": tod_type must be GMM_TOD_F16 or GMM_TOD_BF16");
+    if (reinterpret_cast<uintptr_t>(tod) & 7)
+        fail(std::string(where) + ": a 16-bit TOD must be 8-byte aligned");
+    require_ptr(scale, where, "tod_scale");
+    if (nrows <= 0 || nsamp % nrows != 0)
+        fail(std::string(where) + ": tod_scale has " + std::to_string(nrows)
+             + " rows, which does not divide nsamp=" + std::to_string(nsamp));
+    TodGains g;
+    g.tod_type = (tod_type == GMM_TOD_F16) ? TOD_F16 : TOD_BF16;
+    g.scale = scale;
+    g.scale_nrows = nrows;
+    if (response) {
+        check_response(response, resp_ndet, nsamp, where);
+        g.response = response;
+        g.resp_ndet = resp_ndet;
+    }
+    return g;
+}
+}  // namespace
+
+int gmm_planned_map2tod_q16(void *tod_gpu, int tod_type, const float *tod_scale_gpu, int64_t tod_nrows,
+                            int64_t tod_nsamp, const float *local_map_gpu, int64_t local_map_size,
+                            const float *xpointing_gpu, int64_t xpointing_nsamp,
+                            const float *response_gpu, int64_t resp_ndet,
+                            const gmm_lpix *lpix, const gmm_plan *plan,
+                            int partial_pixelization, int debug, int accum, int check_errors, void *stream_)
+{
+    return guarded([&] {
+        const char *where = "map2tod (16-bit TOD)";
+        hipStream_t stream = as_stream(stream_);
+        require_ptr(plan, where, "plan");
+        require_ptr(tod_gpu, where, "tod");
+        require_ptr(local_map_gpu, where, "local_map");
+        require_ptr(xpointing_gpu, where, "xpointing");
+        check_op_common(lpix, plan, tod_nsamp, xpointing_nsamp, local_map_size, where);
+        TodGains g = q16_gains(tod_type, tod_gpu, tod_scale_gpu, tod_nrows, tod_nsamp, response_gpu, resp_ndet, where);
+        require_ptr(lpix->offsets_gpu, where, "cell_offsets_gpu");
+        OpArgs a{lpix, tod_nsamp, partial_pixelization != 0};
+        launch_planned_map2tod(tod_gpu, local_map_gpu, xpointing_gpu, a, *plan, accum != 0, g, stream);
+        if (debug || check_errors)
+            check_plan_errors(*plan, where, stream);
+    });
+}
+
+int gmm_planned_tod2map_q16(float *local_map_gpu, int64_t local_map_size,
+                            const void *tod_gpu, int tod_type, const float *tod_scale_gpu, int64_t tod_nrows,
+                            int64_t tod_nsamp, const float *xpointing_gpu, int64_t xpointing_nsamp,
+                            const float *response_gpu, int64_t resp_ndet,
+                            const gmm_lpix *lpix, const gmm_plan *plan,
+                            int partial_pixelization, int debug, int check_errors, void *stream_)
+{
+    return guarded([&] {
+        const char *where = "tod2map (16-bit TOD)";
+        hipStream_t stream = as_stream(stream_);
+        require_ptr(plan, where, "plan");
+        require_ptr(tod_gpu, where, "tod");
+        require_ptr(local_map_gpu, where, "local_map");
+        require_ptr(xpointing_gpu, where, "xpointing");
+        check_op_common(lpix, plan, tod_nsamp, xpointing_nsamp, local_map_size, where);
+        TodGains g = q16_gains(tod_type, tod_gpu, tod_scale_gpu, tod_nrows, tod_nsamp, response_gpu, resp_ndet, where);
+        require_ptr(lpix->offsets_gpu, where, "cell_offsets_gpu");
+        OpArgs a{lpix, tod_nsamp, partial_pixelization != 0};
+        launch_planned_tod2map(local_map_gpu, tod_gpu, xpointing_gpu, a, *plan, g, stream);
+        if (debug || check_errors)
+            check_plan_errors(*plan, where, stream);
+    });
+}
+
+int gmm_tod16_encode(void *out_gpu, int tod_type, float *scale_gpu, const float *tod_gpu, int64_t nrows,
+                     int64_t row_len, void *stream_)
+{
+    return guarded([&] {
+        const char *where = "tod16_encode";
+        require_ptr(out_gpu, where, "out");
+        require_ptr(scale_gpu, where, "scale");
+        require_ptr(tod_gpu, where, "tod");
+        if (tod_type != GMM_TOD_F16 && tod_type != GMM_TOD_BF16)
+            fail(std::string(where) + ": tod_type must be GMM_TOD_F16 or GMM_TOD_BF16");
+        if (nrows < 0 || row_len < 0 || nrows > int64_t(INT32_MAX))
+            fail(std::string(where) + ": bad shape");
+        if (nrows == 0 || row_len == 0)
+            return;
+        launch_tod16_encode(out_gpu, tod_type == GMM_TOD_F16 ? TOD_F16 : TOD_BF16, scale_gpu, tod_gpu, nrows,
+                            row_len, as_stream(stream_));
     });
 }
 

@@ -146,10 +146,31 @@ struct OpArgs
     bool partial;
 };
 
-void launch_planned_map2tod(float *tod, const float *lmap, const float *xp, const OpArgs &a,
-                            const gmm_plan &plan, bool accum, hipStream_t stream);
-void launch_planned_tod2map(float *lmap, const float *tod, const float *xp, const OpArgs &a,
-                            const gmm_plan &plan, hipStream_t stream);
+// Kernel timer (measurement aid): event pairs around the main planned kernels while enabled.
+enum TimedKernel { TK_MAP2TOD = 0, TK_TOD2MAP = 1, TK_COUNT = 2 };
+void kernel_timer_mark(int which, bool end, hipStream_t stream);
+
+// TOD element type and per-sample gains of a planned operator call.
+enum TodType { TOD_F32 = 0, TOD_F16 = 1, TOD_BF16 = 2 };
+
+struct TodGains
+{
+    int tod_type = TOD_F32;
+    const float *response = nullptr;   // float[2][resp_ndet] on the device, or null
+    int64_t resp_ndet = 0;
+    const float *scale = nullptr;      // float[scale_nrows] on the device (16-bit TODs), or null
+    int64_t scale_nrows = 0;
+};
+
+void launch_planned_map2tod(void *tod, const float *lmap, const float *xp, const OpArgs &a,
+                            const gmm_plan &plan, bool accum, const TodGains &g, hipStream_t stream);
+void launch_planned_tod2map(float *lmap, const void *tod, const float *xp, const OpArgs &a,
+                            const gmm_plan &plan, const TodGains &g, hipStream_t stream);
+// Per-row 16-bit encoding of a float32 TOD [nrows][row_len]: scale[r] = max(max_s |tod[r][s]| /
+// (0.5 * largest finite value of the type), FLT_MIN), out[r][s] = tod[r][s] / scale[r] rounded
+// to nearest even.
+void launch_tod16_encode(void *out, int tod_type, float *scale, const float *tod, int64_t nrows,
+                         int64_t row_len, hipStream_t stream);
 // Words of tod2map scratch kept in the plan buffer.
 int64_t tod2map_scratch_words();
 

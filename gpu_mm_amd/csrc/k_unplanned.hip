@@ -53,7 +53,7 @@ unplanned_map2tod_kernel(float *__restrict__ tod, const float *__restrict__ lmap
     for (int64_t s = s0 + threadIdx.x; s < s1; s += UP_THREADS) {
         const float y = xp[s], x = xp[s + nsamp], alpha = xp[s + 2 * nsamp];
         float sn, cs;
-        sincosf(2.0f * alpha, &sn, &cs);
+        sincos_angle(2.0f * alpha, sn, cs);
         Pixel<float> p;
         err |= locate_pixel(y, x, ny, nx, Periodic, p);
 
@@ -102,7 +102,7 @@ unplanned_tod2map_kernel(float *__restrict__ lmap, const float *__restrict__ tod
         const float y = xp[s], x = xp[s + nsamp], alpha = xp[s + 2 * nsamp];
         const float t = tod[s];
         float sn, cs;
-        sincosf(2.0f * alpha, &sn, &cs);
+        sincos_angle(2.0f * alpha, sn, cs);
         const float q = t * cs, u = t * sn;
         Pixel<float> p;
         err |= locate_pixel(y, x, ny, nx, Periodic, p);

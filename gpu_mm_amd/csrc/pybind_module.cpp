@@ -41,7 +41,7 @@ struct DLManagedTensor
 };
 
 enum { kDLCPU = 1, kDLCUDA = 2, kDLCUDAHost = 3, kDLROCM = 10, kDLROCMHost = 11 };
-enum { kDLInt = 0, kDLUInt = 1, kDLFloat = 2 };
+enum { kDLInt = 0, kDLUInt = 1, kDLFloat = 2, kDLBfloat = 4 };
 
 [[noreturn]] void raise(const std::string &s) { throw std::runtime_error(s); }
 
@@ -291,6 +291,126 @@ void planned_tod2map(py::object local_map, py::object tod, py::object xpointing,
                               partial_pixelization, debug, check_errors, as_stream(stream)));
 }
 
+// Response array: float32 (2, ndet) on the GPU, C-contiguous (src_lib/check_arguments.cu:171-180).
+Arr response_arr(py::handle o, const char *where)
+{
+    Arr r = get_arr(o, where, "response");
+    require(r, where, "response", kDLFloat, 32, true, "float32");
+    if (r.shape.size() != 2 || r.shape[0] != 2)
+        raise(std::string(where) + ": expected 'response' to have shape (2, ndet)");
+    return r;
+}
+
+void response_map2tod(py::object tod, py::object local_map, py::object xpointing, py::object response,
+                      const LocalPixelization &lpix, const PointingPlan &plan, bool partial_pixelization, bool debug,
+                      bool accum, bool check_errors, uintptr_t stream)
+{
+    const char *where = "response_map2tod";
+    Arr t = tod_arr(tod, where, true), m = map_arr(local_map, where, true), x = xp_arr(xpointing, where, true);
+    Arr r = response_arr(response, where);
+    py::gil_scoped_release nogil;
+    check(gmm_response_map2tod(static_cast<float *>(t.ptr), t.size, static_cast<const float *>(m.ptr), m.size,
+                               static_cast<const float *>(x.ptr), x.size / 3, static_cast<const float *>(r.ptr),
+                               r.shape[1], lpix.h, plan.h, partial_pixelization, debug, accum, check_errors,
+                               as_stream(stream)));
+}
+
+void response_tod2map(py::object local_map, py::object tod, py::object xpointing, py::object response,
+                      const LocalPixelization &lpix, const PointingPlan &plan, bool partial_pixelization, bool debug,
+                      bool check_errors, uintptr_t stream)
+{
+    const char *where = "response_tod2map";
+    Arr m = map_arr(local_map, where, true), t = tod_arr(tod, where, true), x = xp_arr(xpointing, where, true);
+    Arr r = response_arr(response, where);
+    py::gil_scoped_release nogil;
+    check(gmm_response_tod2map(static_cast<float *>(m.ptr), m.size, static_cast<const float *>(t.ptr), t.size,
+                               static_cast<const float *>(x.ptr), x.size / 3, static_cast<const float *>(r.ptr),
+                               r.shape[1], lpix.h, plan.h, partial_pixelization, debug, check_errors,
+                               as_stream(stream)));
+}
+
+// 16-bit TOD: float16 or bfloat16 on the GPU, (nsamp,) or (ndet,nt), with its float32 per-row
+// scale array of shape (nrows,).
+struct Tod16
+{
+    Arr t, sc;
+    int type;
+};
+
+Tod16 tod16_args(py::handle tod, py::handle scale, const char *where)
+{
+    Tod16 r;
+    r.t = get_arr(tod, where, "tod");
+    if (r.t.code == kDLFloat && r.t.bits == 16)
+        r.type = GMM_TOD_F16;
+    else if (r.t.code == kDLBfloat && r.t.bits == 16)
+        r.type = GMM_TOD_BF16;
+    else
+        raise(std::string(where) + ": expected a 16-bit tod to have dtype float16 or bfloat16");
+    if (!r.t.on_gpu)
+        raise(std::string(where) + ": expected tod to be in GPU memory");
+    if (!r.t.contiguous)
+        raise(std::string(where) + ": expected tod to be contiguous");
+    tod_nsamp(r.t, where);
+    r.sc = get_arr(scale, where, "tod_scale");
+    require(r.sc, where, "tod_scale", kDLFloat, 32, true, "float32");
+    if (r.sc.shape.size() != 1)
+        raise(std::string(where) + ": expected tod_scale to be a 1-d array");
+    if (r.t.shape.size() == 2 && r.sc.shape[0] != r.t.shape[0])
+        raise(std::string(where) + ": expected tod_scale to have one entry per TOD row");
+    return r;
+}
+
+void planned_map2tod_q16(py::object tod, py::object tod_scale, py::object local_map, py::object xpointing,
+                         const LocalPixelization &lpix, const PointingPlan &plan, bool partial_pixelization,
+                         bool debug, bool accum, py::object response, bool check_errors, uintptr_t stream)
+{
+    const char *where = "map2tod (16-bit TOD)";
+    Tod16 t = tod16_args(tod, tod_scale, where);
+    Arr m = map_arr(local_map, where, true), x = xp_arr(xpointing, where, true);
+    Arr r;
+    if (!response.is_none())
+        r = response_arr(response, where);
+    py::gil_scoped_release nogil;
+    check(gmm_planned_map2tod_q16(t.t.ptr, t.type, static_cast<const float *>(t.sc.ptr), t.sc.size, t.t.size,
+                                  static_cast<const float *>(m.ptr), m.size, static_cast<const float *>(x.ptr),
+                                  x.size / 3, static_cast<const float *>(r.ptr), r.ptr ? r.shape[1] : 0, lpix.h,
+                                  plan.h, partial_pixelization, debug, accum, check_errors, as_stream(stream)));
+}
+
+void planned_tod2map_q16(py::object local_map, py::object tod, py::object tod_scale, py::object xpointing,
+                         const LocalPixelization &lpix, const PointingPlan &plan, bool partial_pixelization,
+                         bool debug, py::object response, bool check_errors, uintptr_t stream)
+{
+    const char *where = "tod2map (16-bit TOD)";
+    Tod16 t = tod16_args(tod, tod_scale, where);
+    Arr m = map_arr(local_map, where, true), x = xp_arr(xpointing, where, true);
+    Arr r;
+    if (!response.is_none())
+        r = response_arr(response, where);
+    py::gil_scoped_release nogil;
+    check(gmm_planned_tod2map_q16(static_cast<float *>(m.ptr), m.size, t.t.ptr, t.type,
+                                  static_cast<const float *>(t.sc.ptr), t.sc.size, t.t.size,
+                                  static_cast<const float *>(x.ptr), x.size / 3, static_cast<const float *>(r.ptr),
+                                  r.ptr ? r.shape[1] : 0, lpix.h, plan.h, partial_pixelization, debug, check_errors,
+                                  as_stream(stream)));
+}
+
+void tod16_encode(py::object out, py::object scale, py::object tod, uintptr_t stream)
+{
+    const char *where = "tod16_encode";
+    Tod16 o = tod16_args(out, scale, where);
+    Arr t = tod_arr(tod, where, true);
+    if (t.shape != o.t.shape)
+        raise(std::string(where) + ": expected out and tod to have the same shape");
+    const int64_t nrows = (t.shape.size() == 2) ? t.shape[0] : 1;
+    if (o.sc.size != nrows)
+        raise(std::string(where) + ": expected scale to have one entry per TOD row");
+    py::gil_scoped_release nogil;
+    check(gmm_tod16_encode(o.t.ptr, o.type, static_cast<float *>(o.sc.ptr), static_cast<const float *>(t.ptr), nrows,
+                           nrows ? t.size / nrows : 0, as_stream(stream)));
+}
+
 Arr errflags_arr(py::handle o, const char *where)
 {
     Arr e = get_arr(o, where, "errflags");
@@ -449,6 +569,27 @@ PYBIND11_MODULE(_gpu_mm_amd_py, m)
     m.def("planned_tod2map", &planned_tod2map, py::arg("local_map"), py::arg("tod"), py::arg("xpointing"),
           py::arg("local_pixelization"), py::arg("plan"), py::arg("partial_pixelization"), py::arg("debug"),
           py::arg("check_errors") = true, py::arg("stream") = 0);
+    m.def("response_map2tod", &response_map2tod, py::arg("tod"), py::arg("local_map"), py::arg("xpointing"),
+          py::arg("response"), py::arg("local_pixelization"), py::arg("plan"), py::arg("partial_pixelization"),
+          py::arg("debug"), py::arg("accum"), py::arg("check_errors") = true, py::arg("stream") = 0);
+    m.def("response_tod2map", &response_tod2map, py::arg("local_map"), py::arg("tod"), py::arg("xpointing"),
+          py::arg("response"), py::arg("local_pixelization"), py::arg("plan"), py::arg("partial_pixelization"),
+          py::arg("debug"), py::arg("check_errors") = true, py::arg("stream") = 0);
+    m.def("planned_map2tod_q16", &planned_map2tod_q16, py::arg("tod"), py::arg("tod_scale"), py::arg("local_map"),
+          py::arg("xpointing"), py::arg("local_pixelization"), py::arg("plan"), py::arg("partial_pixelization"),
+          py::arg("debug"), py::arg("accum"), py::arg("response") = py::none(), py::arg("check_errors") = true,
+          py::arg("stream") = 0);
+    m.def("planned_tod2map_q16", &planned_tod2map_q16, py::arg("local_map"), py::arg("tod"), py::arg("tod_scale"),
+          py::arg("xpointing"), py::arg("local_pixelization"), py::arg("plan"), py::arg("partial_pixelization"),
+          py::arg("debug"), py::arg("response") = py::none(), py::arg("check_errors") = true, py::arg("stream") = 0);
+    m.def("kernel_timer_enable", [](bool on) { check(gmm_kernel_timer_enable(on ? 1 : 0)); });
+    m.def("kernel_timer_read", [](int which) {
+        double ms = 0.0;
+        int64_t n = 0;
+        check(gmm_kernel_timer_read(which, &ms, &n));
+        return py::make_tuple(ms, n);
+    }, "Returns (total_ms, launches) of GMM_TIMED_MAP2TOD (0) or GMM_TIMED_TOD2MAP (1) since the last enable/read");
+    m.def("tod16_encode", &tod16_encode, py::arg("out"), py::arg("scale"), py::arg("tod"), py::arg("stream") = 0);
     m.def("unplanned_map2tod", &unplanned_map2tod, py::arg("tod"), py::arg("local_map"), py::arg("xpointing"),
           py::arg("local_pixelization"), py::arg("errflags"), py::arg("partial_pixelization"), py::arg("accum"),
           py::arg("check_errors") = true, py::arg("stream") = 0);

@@ -72,16 +72,43 @@ def _is_gpu_array(a):
     return isinstance(a, torch.Tensor) and a.is_cuda
 
 
-def map2tod(tod, local_map, xpointing, plan, partial_pixelization=False, response=None, debug=False, accum=False):
+def _is_tod16(tod):
+    return isinstance(tod, torch.Tensor) and tod.dtype in (torch.float16, torch.bfloat16)
+
+
+def encode_tod16(tod, dtype=torch.float16):
+    """
+    Encodes a float32 GPU TOD of shape (nsamp,) or (ndet, nt) as a 16-bit TOD (float16 or
+    bfloat16) with one float32 scale per row: scale = max(max|row| / (0.5 * largest finite
+    value of dtype), FLT_MIN), tod16 = tod / scale rounded to nearest even.
+
+    Returns (tod16, scale); decode with tod16.float() * scale[:, None] (or * scale for 1-d).
+    Not in the reference.
+    """
+    if dtype not in (torch.float16, torch.bfloat16):
+        raise RuntimeError(f"encode_tod16(): dtype must be torch.float16 or torch.bfloat16, got {dtype}")
+    nrows = tod.shape[0] if tod.ndim == 2 else 1
+    out = torch.empty(tod.shape, dtype=dtype, device=tod.device)
+    scale = torch.empty(nrows, dtype=torch.float32, device=tod.device)
+    _ext.tod16_encode(out, scale, tod, stream=_stream())
+    return out, scale
+
+
+def map2tod(tod, local_map, xpointing, plan, partial_pixelization=False, response=None, debug=False, accum=False,
+            tod_scale=None):
     """
     tod (=, or += if accum) A * local_map.
 
-      - tod: float32 array of shape (nsamp,) or (ndet, nt); output.
+      - tod: float32 array of shape (nsamp,) or (ndet, nt); output. With a PointingPlan it may
+        also be a float16 / bfloat16 GPU tensor that stores value / tod_scale[row] (see
+        encode_tod16); then tod_scale (float32, one entry per row) is required.
       - local_map: LocalMap.
       - xpointing: float32 array of shape (3, nsamp) or (3, ndet, nt).
       - plan: PointingPlan, None (plan-free GPU kernels) or 'reference' (CPU, host arrays).
       - partial_pixelization: if True, pointing outside the local map reads zeros instead of
         raising.
+      - response: optional float32 array of shape (2, ndet) on the GPU (PointingPlan only): row 0
+        multiplies T, row 1 multiplies Q and U, for every sample of detector i = s // (nsamp/ndet).
       - accum: add into 'tod' instead of overwriting it.
 
     Reference: gpu_mm/gpu_mm.py:262-312.
@@ -90,12 +117,23 @@ def map2tod(tod, local_map, xpointing, plan, partial_pixelization=False, respons
         raise RuntimeError(f"map2tod(): expected LocalMap, got {local_map!r}")
     lpix = local_map.pixelization
     larr = local_map.arr
-    if response is not None:
-        raise RuntimeError("map2tod(): per-detector 'response' is not supported by this build")
+    if (response is not None) and not isinstance(plan, PointingPlan):
+        # The reference silently ignores 'response' here (gpu_mm.py:305-310); an error is safer
+        # than a result without the response applied.
+        raise RuntimeError("map2tod(): 'response' requires a PointingPlan")
 
-    if isinstance(plan, PointingPlan):
-        _ext.planned_map2tod(tod, larr, xpointing, lpix, plan, partial_pixelization, debug, accum,
-                             check_errors=_errflag_check, stream=_stream())
+    if _is_tod16(tod) or (tod_scale is not None):
+        if not (isinstance(plan, PointingPlan) and _is_tod16(tod) and tod_scale is not None):
+            raise RuntimeError("map2tod(): a 16-bit TOD needs a PointingPlan and tod_scale")
+        _ext.planned_map2tod_q16(tod, tod_scale, larr, xpointing, lpix, plan, partial_pixelization, debug, accum,
+                                 response=response, check_errors=_errflag_check, stream=_stream())
+    elif isinstance(plan, PointingPlan):
+        if response is None:
+            _ext.planned_map2tod(tod, larr, xpointing, lpix, plan, partial_pixelization, debug, accum,
+                                 check_errors=_errflag_check, stream=_stream())
+        else:
+            _ext.response_map2tod(tod, larr, xpointing, response, lpix, plan, partial_pixelization, debug,
+                                  accum, check_errors=_errflag_check, stream=_stream())
     elif plan is None:
         errflags = torch.zeros(_UNPLANNED_MAX_BLOCKS, dtype=torch.uint32, device=_device())
         _ext.unplanned_map2tod(tod, larr, xpointing, lpix, errflags, partial_pixelization, accum,
@@ -106,23 +144,34 @@ def map2tod(tod, local_map, xpointing, plan, partial_pixelization=False, respons
         raise RuntimeError(f"Bad 'plan' argument to map2tod(): {plan}")
 
 
-def tod2map(local_map, tod, xpointing, plan, partial_pixelization=False, response=None, debug=False):
+def tod2map(local_map, tod, xpointing, plan, partial_pixelization=False, response=None, debug=False,
+            tod_scale=None):
     """
     local_map += A^T * tod.
 
-    Arguments as in map2tod(); the output map is always accumulated into.
+    Arguments as in map2tod() (including 16-bit TODs with tod_scale); the output map is always
+    accumulated into.
     Reference: gpu_mm/gpu_mm.py:314-385.
     """
     if not isinstance(local_map, LocalMap):
         raise RuntimeError(f"tod2map(): Bad 'local_map' argument to tod2map(): expected LocalMap, got: {local_map}")
     lpix = local_map.pixelization
     larr = local_map.arr
-    if response is not None:
-        raise RuntimeError("tod2map(): per-detector 'response' is not supported by this build")
+    if (response is not None) and not isinstance(plan, PointingPlan):
+        raise RuntimeError("tod2map(): 'response' requires a PointingPlan")
 
-    if isinstance(plan, PointingPlan):
-        _ext.planned_tod2map(larr, tod, xpointing, lpix, plan, partial_pixelization, debug,
-                             check_errors=_errflag_check, stream=_stream())
+    if _is_tod16(tod) or (tod_scale is not None):
+        if not (isinstance(plan, PointingPlan) and _is_tod16(tod) and tod_scale is not None):
+            raise RuntimeError("tod2map(): a 16-bit TOD needs a PointingPlan and tod_scale")
+        _ext.planned_tod2map_q16(larr, tod, tod_scale, xpointing, lpix, plan, partial_pixelization, debug,
+                                 response=response, check_errors=_errflag_check, stream=_stream())
+    elif isinstance(plan, PointingPlan):
+        if response is None:
+            _ext.planned_tod2map(larr, tod, xpointing, lpix, plan, partial_pixelization, debug,
+                                 check_errors=_errflag_check, stream=_stream())
+        else:
+            _ext.response_tod2map(larr, tod, xpointing, response, lpix, plan, partial_pixelization, debug,
+                                  check_errors=_errflag_check, stream=_stream())
     elif plan is None:
         errflags = torch.zeros(_UNPLANNED_MAX_BLOCKS, dtype=torch.uint32, device=_device())
         _ext.unplanned_tod2map(larr, tod, xpointing, lpix, errflags, partial_pixelization,

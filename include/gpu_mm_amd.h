@@ -155,6 +155,67 @@ int gmm_planned_tod2map(float *local_map_gpu, int64_t local_map_size,
                         const gmm_lpix *lpix, const gmm_plan *plan,
                         int partial_pixelization, int debug, int check_errors, void *stream);
 
+/* launch_response_map2tod, include/gpu_mm.hpp:179-190; src_lib/map2tod_response.cu:205-280.
+ * response_gpu: float[2][ndet] (row 0 scales T, row 1 scales Q and U); sample s belongs to
+ * detector s / (nsamp/ndet); ndet must divide nsamp. Otherwise as gmm_planned_map2tod. */
+int gmm_response_map2tod(float *tod_gpu, int64_t tod_nsamp,
+                         const float *local_map_gpu, int64_t local_map_size,
+                         const float *xpointing_gpu, int64_t xpointing_nsamp,
+                         const float *response_gpu, int64_t ndet,
+                         const gmm_lpix *lpix, const gmm_plan *plan,
+                         int partial_pixelization, int debug, int accum, int check_errors,
+                         void *stream);
+
+/* launch_response_tod2map, include/gpu_mm.hpp:192-202; src_lib/tod2map_response.cu:165-240.
+ * Deposits response[0][det]*d into T and response[1][det]*d*{cos,sin}(2 alpha) into Q, U. */
+int gmm_response_tod2map(float *local_map_gpu, int64_t local_map_size,
+                         const float *tod_gpu, int64_t tod_nsamp,
+                         const float *xpointing_gpu, int64_t xpointing_nsamp,
+                         const float *response_gpu, int64_t ndet,
+                         const gmm_lpix *lpix, const gmm_plan *plan,
+                         int partial_pixelization, int debug, int check_errors, void *stream);
+
+/* ---- Kernel timer -------------------------------------------------------------------------
+ * No counterpart in the reference; a measurement aid. While enabled, every planned operator
+ * records a HIP event pair on its launch stream around its main kernel (planned_map2tod_kernel
+ * = GMM_TIMED_MAP2TOD, planned_tod2map_kernel = GMM_TIMED_TOD2MAP). enable() resets the counts;
+ * read() waits for the recorded events and returns the summed milliseconds and the number of
+ * launches since the last enable/read, then resets that kernel's count. */
+enum { GMM_TIMED_MAP2TOD = 0, GMM_TIMED_TOD2MAP = 1 };
+int gmm_kernel_timer_enable(int enable);
+int gmm_kernel_timer_read(int which, double *total_ms, int64_t *count);
+
+/* ---- 16-bit TODs -------------------------------------------------------------------------
+ * No counterpart in the reference (which stores TODs as float32 only). A 16-bit TOD of nsamp
+ * samples in tod_nrows rows holds value / tod_scale[row] as FP16 or BF16, with one FP32 scale
+ * per row; the operators compute in FP32 exactly as the float32 ones do. tod_gpu must be 8-byte
+ * aligned. response_gpu may be null, or float[2][resp_ndet] as in gmm_response_map2tod. */
+enum { GMM_TOD_F16 = 1, GMM_TOD_BF16 = 2 };
+
+/* map2tod into a 16-bit TOD: each output is rounded once to the 16-bit type, except on lines
+ * that touch more than one cell, whose per-cell partials are rounded and then added with packed
+ * 16-bit atomics. */
+int gmm_planned_map2tod_q16(void *tod_gpu, int tod_type, const float *tod_scale_gpu, int64_t tod_nrows,
+                            int64_t tod_nsamp, const float *local_map_gpu, int64_t local_map_size,
+                            const float *xpointing_gpu, int64_t xpointing_nsamp,
+                            const float *response_gpu, int64_t resp_ndet,
+                            const gmm_lpix *lpix, const gmm_plan *plan,
+                            int partial_pixelization, int debug, int accum, int check_errors, void *stream);
+
+/* tod2map from a 16-bit TOD (samples decoded as value * scale in FP32, accumulated exactly). */
+int gmm_planned_tod2map_q16(float *local_map_gpu, int64_t local_map_size,
+                            const void *tod_gpu, int tod_type, const float *tod_scale_gpu, int64_t tod_nrows,
+                            int64_t tod_nsamp, const float *xpointing_gpu, int64_t xpointing_nsamp,
+                            const float *response_gpu, int64_t resp_ndet,
+                            const gmm_lpix *lpix, const gmm_plan *plan,
+                            int partial_pixelization, int debug, int check_errors, void *stream);
+
+/* Encodes a float32 TOD [nrows][row_len] (device) into out_gpu (16-bit, same shape) and
+ * scale_gpu (float[nrows]): scale = max(max|row| / (0.5 * largest finite value), FLT_MIN),
+ * out = tod / scale rounded to nearest even. */
+int gmm_tod16_encode(void *out_gpu, int tod_type, float *scale_gpu, const float *tod_gpu, int64_t nrows,
+                     int64_t row_len, void *stream);
+
 /* launch_unplanned_map2tod, include/gpu_mm.hpp:204-213; src_lib/map2tod_unplanned.cu:74-127.
  * errflags_gpu: caller-supplied uint32[nerrflags]; the launch uses at most nerrflags blocks. */
 int gmm_unplanned_map2tod(float *tod_gpu, int64_t tod_nsamp,

@@ -245,6 +245,63 @@ uint32_t orc_tod2map_f64(double *lmap, const double *tod, const float *xp, int64
     return err;
 }
 
+uint32_t orc_response_map2tod_f64(double *tod, const double *lmap, const float *xp, const float *resp,
+                                  int64_t ndet, int64_t nsamp, const orc_pix *p, int partial, int accum)
+{
+    const int64_t nper = nsamp / ndet;
+    uint32_t err = 0;
+    for (int64_t s = 0; s < nsamp; s++) {
+        const double tr = resp[s / nper], pr = resp[ndet + s / nper];
+        double sn, cs;
+        sincos(2.0 * (double)xp[s + 2 * nsamp], &sn, &cs);
+        int c[4];
+        double d[2];
+        err |= orc_locate_f64(xp[s], xp[s + nsamp], p->ny, p->nx, p->periodic, c, d);
+        const int ys[4] = { c[0], c[0], c[1], c[1] };
+        const int xs[4] = { c[2], c[3], c[2], c[3] };
+        const double w[4] = { (1 - d[0]) * (1 - d[1]), (1 - d[0]) * d[1], d[0] * (1 - d[1]), d[0] * d[1] };
+        double r = 0.0;
+        for (int k = 0; k < 4; k++) {
+            int64_t o = pix_offset(p, ys[k], xs[k]);
+            if (o >= 0)
+                r += w[k] * (tr * lmap[o] + pr * (cs * lmap[o + p->polstride] + sn * lmap[o + 2 * p->polstride]));
+            else if (!partial)
+                err |= ORC_NOT_IN_PIX;
+        }
+        tod[s] = accum ? (tod[s] + r) : r;
+    }
+    return err;
+}
+
+uint32_t orc_response_tod2map_f64(double *lmap, const double *tod, const float *xp, const float *resp,
+                                  int64_t ndet, int64_t nsamp, const orc_pix *p, int partial)
+{
+    const int64_t nper = nsamp / ndet;
+    uint32_t err = 0;
+    for (int64_t s = 0; s < nsamp; s++) {
+        const double tt = tod[s] * resp[s / nper], tp = tod[s] * resp[ndet + s / nper];
+        double sn, cs;
+        sincos(2.0 * (double)xp[s + 2 * nsamp], &sn, &cs);
+        int c[4];
+        double d[2];
+        err |= orc_locate_f64(xp[s], xp[s + nsamp], p->ny, p->nx, p->periodic, c, d);
+        const int ys[4] = { c[0], c[0], c[1], c[1] };
+        const int xs[4] = { c[2], c[3], c[2], c[3] };
+        const double w[4] = { (1 - d[0]) * (1 - d[1]), (1 - d[0]) * d[1], d[0] * (1 - d[1]), d[0] * d[1] };
+        for (int k = 0; k < 4; k++) {
+            int64_t o = pix_offset(p, ys[k], xs[k]);
+            if (o >= 0) {
+                lmap[o] += w[k] * tt;
+                lmap[o + p->polstride] += w[k] * tp * cs;
+                lmap[o + 2 * p->polstride] += w[k] * tp * sn;
+            }
+            else if (!partial)
+                err |= ORC_NOT_IN_PIX;
+        }
+    }
+    return err;
+}
+
 /* ---------------------------------------------------------------------------------------
  * Multi-threaded FP32 (CPU baseline on all cores).
  * --------------------------------------------------------------------------------------- */

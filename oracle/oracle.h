@@ -66,6 +66,14 @@ uint32_t orc_map2tod_f64(double *tod, const double *lmap, const float *xp, int64
 uint32_t orc_tod2map_f64(double *lmap, const double *tod, const float *xp, int64_t nsamp,
                          const orc_pix *p, int partial);
 
+/* response_map2tod / response_tod2map (src_lib/map2tod_response.cu:81-198,
+ * src_lib/tod2map_response.cu:20-160) in FP64: resp[0..ndet) scales T, resp[ndet..2 ndet)
+ * scales Q and U, for the samples of detector s / (nsamp/ndet). */
+uint32_t orc_response_map2tod_f64(double *tod, const double *lmap, const float *xp, const float *resp,
+                                  int64_t ndet, int64_t nsamp, const orc_pix *p, int partial, int accum);
+uint32_t orc_response_tod2map_f64(double *lmap, const double *tod, const float *xp, const float *resp,
+                                  int64_t ndet, int64_t nsamp, const orc_pix *p, int partial);
+
 /* Multi-threaded FP32 variants (OpenMP; tod2map with per-thread private maps then a
  * reduction), used as the N-core CPU baseline. nthreads <= 0 uses all cores. */
 uint32_t orc_map2tod_f32_mt(float *tod, const float *lmap, const float *xp, int64_t nsamp,

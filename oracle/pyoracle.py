@@ -46,6 +46,10 @@ def lib():
         for name in ("orc_tod2map_f32", "orc_tod2map_f64"):
             getattr(L, name).argtypes = [_vp, _vp, _vp, _i64, ctypes.POINTER(_Pix), _int]
             getattr(L, name).restype = _u32
+        L.orc_response_map2tod_f64.argtypes = [_vp, _vp, _vp, _vp, _i64, _i64, ctypes.POINTER(_Pix), _int, _int]
+        L.orc_response_map2tod_f64.restype = _u32
+        L.orc_response_tod2map_f64.argtypes = [_vp, _vp, _vp, _vp, _i64, _i64, ctypes.POINTER(_Pix), _int]
+        L.orc_response_tod2map_f64.restype = _u32
         L.orc_map2tod_f32_mt.argtypes = [_vp, _vp, _vp, _i64, ctypes.POINTER(_Pix), _int, _int, _int]
         L.orc_map2tod_f32_mt.restype = _u32
         L.orc_tod2map_f32_mt.argtypes = [_vp, _i64, _vp, _vp, _i64, ctypes.POINTER(_Pix), _int, _int]
@@ -129,6 +133,34 @@ def tod2map(tod, xp, pix, lmap=None, partial=False, f64=False, nthreads=None):
         err = lib().orc_tod2map_f64(_ptr(out), _ptr(tod), _ptr(xp), n, ctypes.byref(p), int(partial))
     else:
         err = lib().orc_tod2map_f32(_ptr(out), _ptr(tod), _ptr(xp), n, ctypes.byref(p), int(partial))
+    return out, int(err)
+
+
+def response_map2tod(lmap, xp, resp, pix, partial=False, tod=None, accum=False):
+    """FP64 response map2tod; resp has shape (2, ndet)."""
+    xp = np.ascontiguousarray(xp, dtype=np.float32).reshape(3, -1)
+    n = xp.shape[1]
+    resp = np.ascontiguousarray(resp, dtype=np.float32)
+    assert resp.ndim == 2 and resp.shape[0] == 2 and n % resp.shape[1] == 0
+    lmap = np.ascontiguousarray(lmap, dtype=np.float64).reshape(-1)
+    out = np.zeros(n) if tod is None else np.array(tod, dtype=np.float64).reshape(-1)
+    p = pix.c()
+    err = lib().orc_response_map2tod_f64(_ptr(out), _ptr(lmap), _ptr(xp), _ptr(resp), resp.shape[1], n,
+                                         ctypes.byref(p), int(partial), int(accum))
+    return out, int(err)
+
+
+def response_tod2map(tod, xp, resp, pix, lmap=None, partial=False):
+    """FP64 response tod2map; resp has shape (2, ndet)."""
+    xp = np.ascontiguousarray(xp, dtype=np.float32).reshape(3, -1)
+    n = xp.shape[1]
+    resp = np.ascontiguousarray(resp, dtype=np.float32)
+    assert resp.ndim == 2 and resp.shape[0] == 2 and n % resp.shape[1] == 0
+    tod = np.ascontiguousarray(tod, dtype=np.float64).reshape(-1)
+    out = np.zeros(3 * pix.npix) if lmap is None else np.array(lmap, dtype=np.float64).reshape(-1)
+    p = pix.c()
+    err = lib().orc_response_tod2map_f64(_ptr(out), _ptr(tod), _ptr(xp), _ptr(resp), resp.shape[1], n,
+                                         ctypes.byref(p), int(partial))
     return out, int(err)
 
 

@@ -276,3 +276,242 @@ def test_shared_plan_buffers(gmm, dev):
     a2 = gmm.PointingPlan(p0, x0)
     assert np.array_equal(a.get_plan_mt(), a2.get_plan_mt())
     assert b.plan_nmt == p1.plan_nmt
+
+
+def test_large_angles(gmm, orc, dev):
+    """The device sin/cos of 2*alpha: polynomial path (|2a| <= 1e6) and double reduction beyond."""
+    rng = np.random.default_rng(11)
+    ny, nx = 128, 256
+    n = 32 * 4096
+    t = np.arange(n)
+    alpha = np.concatenate([rng.uniform(-3, 3, n // 4), rng.uniform(-4e5, 4e5, n // 4),
+                            rng.uniform(-6e6, 6e6, n // 4), rng.uniform(-1e9, 1e9, n // 4)])
+    xp = np.stack([64 + 50 * np.sin(t / 500.0), 128 + 100 * np.sin(t / 700.0), alpha]).astype(F32)
+    lp = gmm.LocalPixelization.make_rectangle(ny, nx, periodic_xcoord=False)
+    pix = orc.Pix.rectangle(ny, nx, False)
+    m = rng.standard_normal(3 * lp.npix).astype(F32)
+    m[:lp.npix] = 0.0          # T = 0: the TOD is pure Q cos + U sin, so angle errors show fully
+    xg = to_gpu(xp, dev)
+    plan = gmm.PointingPlan(gmm.PointingPrePlan(xg, ny, nx, periodic_xcoord=False), xg)
+    ref32, err = orc.map2tod(m, xp, pix)
+    assert err == 0
+    ref64, _ = orc.map2tod(m.astype(np.float64), xp, pix, f64=True)
+    tg = torch.zeros(n, dtype=torch.float32, device=dev)
+    for p in (plan, None):
+        gmm.map2tod(tg, gmm.LocalMap(lp, to_gpu(m, dev)), xg, p)
+        got = to_cpu(tg)
+        for sl in (slice(0, n // 4), slice(n // 4, n // 2), slice(n // 2, 3 * n // 4), slice(3 * n // 4, n)):
+            assert l1rel(got[sl], ref32[sl]) < 1e-6
+            assert l1rel(got[sl], ref64[sl]) < 1e-5
+
+
+# ------------------------------------------------------------------------------------------
+# Per-detector response (src_lib/map2tod_response.cu, tod2map_response.cu).
+
+def test_response_ops(inst, gmm, orc, dev):
+    rng = np.random.default_rng(400 + inst["seed"])
+    lp, plan = inst["lp"], inst["plan"]
+    ndet = inst["xp"].shape[1]
+    pix = opix(orc, inst)
+    resp = rng.uniform(-2, 2, size=(2, ndet)).astype(F32)
+    rg = to_gpu(resp, dev)
+    m = rng.standard_normal(3 * lp.npix).astype(F32)
+    tod0 = rng.standard_normal(inst["xp"].shape[1:]).astype(F32)
+    mg = gmm.LocalMap(lp, to_gpu(m, dev))
+    for accum in (False, True):
+        ref, err = orc.response_map2tod(m, inst["xp"], resp, pix, tod=tod0, accum=accum)
+        assert err == 0
+        t = to_gpu(tod0, dev)
+        gmm.map2tod(t, mg, inst["xg"], plan, response=rg, accum=accum, debug=True)
+        e = l1rel(to_cpu(t), ref)
+        print(f"  response map2tod accum={accum}: l1rel vs f64 oracle {e:.2e}")
+        assert e < 1e-6
+    m0 = rng.standard_normal(3 * lp.npix).astype(F32)
+    ref, err = orc.response_tod2map(tod0, inst["xp"], resp, pix, lmap=m0)
+    assert err == 0
+    out = gmm.LocalMap(lp, to_gpu(m0, dev))
+    gmm.tod2map(out, to_gpu(tod0, dev), inst["xg"], plan, response=rg, debug=True)
+    e = l1rel(to_cpu(out.arr), ref)
+    print(f"  response tod2map: l1rel vs f64 oracle {e:.2e}")
+    assert e < 1e-6
+
+
+def test_response_reference_check(inst, gmm, orc, dev):
+    """gpu_mm/tests.py:295-315: response (2, -1) with the map divided by (2, -1) gives the plain TOD."""
+    rng = np.random.default_rng(500 + inst["seed"])
+    lp, plan = inst["lp"], inst["plan"]
+    ndet = inst["xp"].shape[1]
+    m = rng.standard_normal(3 * lp.npix).astype(F32)
+    ref, _ = orc.map2tod(m, inst["xp"], opix(orc, inst))
+    m2 = m.copy()
+    m2[:lp.npix] /= 2
+    m2[lp.npix:] /= -1
+    resp = torch.zeros((2, ndet), dtype=torch.float32, device=dev)
+    resp[0], resp[1] = 2, -1
+    t = torch.zeros(inst["xp"].shape[1:], dtype=torch.float32, device=dev)
+    gmm.map2tod(t, gmm.LocalMap(lp, to_gpu(m2, dev)), inst["xg"], plan, response=resp, debug=True)
+    assert l1rel(to_cpu(t), ref) < 1e-6
+
+
+@pytest.mark.parametrize("ndet", [2240, 20160, 630])
+def test_response_detector_boundaries(gmm, orc, dev, ndet):
+    """nsamp/ndet = 9, 1 and 32 samples per detector: detector boundaries inside a 4-sample group."""
+    rng = np.random.default_rng(ndet)
+    ny, nx, n = 200, 300, 20160
+    t = np.arange(n)
+    xp = np.stack([100 + 90 * np.sin(t / 300.0), 150 + 140 * np.sin(t / 410.0),
+                   rng.uniform(-3, 3, n)]).astype(F32)
+    lp = gmm.LocalPixelization.make_rectangle(ny, nx, periodic_xcoord=False)
+    pix = orc.Pix.rectangle(ny, nx, False)
+    xg = to_gpu(xp, dev)
+    plan = gmm.PointingPlan(gmm.PointingPrePlan(xg, ny, nx, periodic_xcoord=False), xg)
+    resp = rng.uniform(-2, 2, size=(2, ndet)).astype(F32)
+    m = rng.standard_normal(3 * lp.npix).astype(F32)
+    d = rng.standard_normal(n).astype(F32)
+    tg = torch.zeros(n, dtype=torch.float32, device=dev)
+    gmm.map2tod(tg, gmm.LocalMap(lp, to_gpu(m, dev)), xg, plan, response=to_gpu(resp, dev))
+    ref, _ = orc.response_map2tod(m, xp, resp, pix)
+    assert l1rel(to_cpu(tg), ref) < 1e-6
+    out = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
+    gmm.tod2map(out, to_gpu(d, dev), xg, plan, response=to_gpu(resp, dev))
+    ref, _ = orc.response_tod2map(d, xp, resp, pix)
+    assert l1rel(to_cpu(out.arr), ref) < 1e-6
+
+
+def test_response_argument_errors(gmm, dev):
+    ny, nx, n = 128, 128, 4096
+    xp = np.stack([np.full(n, 60.0), np.linspace(10, 100, n), np.zeros(n)]).astype(F32)
+    xg = to_gpu(xp, dev)
+    lp = gmm.LocalPixelization.make_rectangle(ny, nx, periodic_xcoord=False)
+    plan = gmm.PointingPlan(gmm.PointingPrePlan(xg, ny, nx, periodic_xcoord=False), xg)
+    mg = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
+    tg = torch.zeros(n, dtype=torch.float32, device=dev)
+    with pytest.raises(RuntimeError, match="shape"):
+        gmm.map2tod(tg, mg, xg, plan, response=torch.ones((3, 4), device=dev))
+    with pytest.raises(RuntimeError, match="divide"):
+        gmm.map2tod(tg, mg, xg, plan, response=torch.ones((2, 3), device=dev))
+    with pytest.raises(RuntimeError, match="divide"):
+        gmm.tod2map(mg, tg, xg, plan, response=torch.ones((2, 3), device=dev))
+    with pytest.raises(RuntimeError, match="PointingPlan"):
+        gmm.map2tod(tg, mg, xg, None, response=torch.ones((2, 4), device=dev))
+    gmm.map2tod(tg, mg, xg, plan, response=torch.ones((2, 4), device=dev))
+
+
+# ------------------------------------------------------------------------------------------
+# 16-bit TODs (FP16 / BF16 values with one FP32 scale per row). Not in the reference.
+
+TOD16 = [(torch.float16, 32752.0, 2.0 ** -11), (torch.bfloat16, float.fromhex("0x1.fep126"), 2.0 ** -8)]
+
+
+def _decode(t16, scale):
+    return (t16.float() * scale[:, None]).cpu().numpy()
+
+
+@pytest.mark.parametrize("dtype,half_max,eps", TOD16, ids=["f16", "bf16"])
+def test_tod16_encode(gmm, dev, dtype, half_max, eps):
+    rng = np.random.default_rng(31)
+    tod = rng.standard_normal((37, 1000)).astype(F32) * np.exp(rng.uniform(-20, 20, (37, 1)).astype(F32))
+    tod[5] = 0.0
+    tg = to_gpu(tod, dev)
+    t16, sc = gmm.encode_tod16(tg, dtype)
+    assert t16.dtype == dtype and sc.dtype == torch.float32 and sc.shape == (37,)
+    expect = np.maximum(np.abs(tod).max(axis=1) / F32(half_max), np.finfo(F32).tiny).astype(F32)
+    assert np.array_equal(to_cpu(sc), expect)
+    dec = _decode(t16, sc)
+    assert np.all(np.abs(dec - tod) <= eps * np.abs(tod) + 1e-45)
+    assert np.all(dec[5] == 0.0)
+
+
+@pytest.mark.parametrize("dtype,half_max,eps", TOD16, ids=["f16", "bf16"])
+def test_tod16_ops(inst, gmm, orc, dev, dtype, half_max, eps):
+    rng = np.random.default_rng(600 + inst["seed"])
+    lp, plan = inst["lp"], inst["plan"]
+    pix = opix(orc, inst)
+    shape = inst["xp"].shape[1:]
+    tod = rng.standard_normal(shape).astype(F32)
+    t16, sc = gmm.encode_tod16(to_gpu(tod, dev), dtype)
+    dec = _decode(t16, sc)
+
+    # tod2map on the decoded samples: the same FP32 inputs as the float32 operator.
+    m0 = rng.standard_normal(3 * lp.npix).astype(F32)
+    ref, err = orc.tod2map(dec.astype(np.float64), inst["xp"], pix, lmap=m0.astype(np.float64), f64=True)
+    assert err == 0
+    out = gmm.LocalMap(lp, to_gpu(m0, dev))
+    gmm.tod2map(out, t16, inst["xg"], plan, tod_scale=sc, debug=True)
+    e = l1rel(to_cpu(out.arr), ref)
+    print(f"  tod2map {dtype}: l1rel vs f64 oracle on decoded TOD {e:.2e}")
+    assert e < 1e-6
+
+    # map2tod: outputs are value / scale rounded to the 16-bit type (twice on multi-cell lines).
+    m = rng.standard_normal(3 * lp.npix).astype(F32)
+    mg = gmm.LocalMap(lp, to_gpu(m, dev))
+    ref, _ = orc.map2tod(m.astype(np.float64), inst["xp"], pix, f64=True)
+    o16 = torch.empty_like(t16)
+    gmm.map2tod(o16, mg, inst["xg"], plan, tod_scale=sc, debug=True)
+    got = _decode(o16, sc).reshape(-1)
+    e = l1rel(got, ref)
+    print(f"  map2tod {dtype}: l1rel vs f64 oracle {e:.2e}")
+    assert e < eps
+
+    # accum: adds into the existing 16-bit values.
+    base = o16.clone()
+    gmm.map2tod(o16, mg, inst["xg"], plan, tod_scale=sc, accum=True)
+    got2 = _decode(o16, sc).reshape(-1)
+    assert l1rel(got2, 2 * ref) < 2 * eps
+
+
+@pytest.mark.parametrize("dtype,half_max,eps", TOD16, ids=["f16", "bf16"])
+def test_tod16_response(gmm, orc, dev, dtype, half_max, eps):
+    rng = np.random.default_rng(77)
+    ny, nx, ndet, nt = 200, 300, 24, 840
+    n = ndet * nt
+    t = np.arange(n)
+    xp = np.stack([100 + 90 * np.sin(t / 300.0), 150 + 140 * np.sin(t / 410.0),
+                   rng.uniform(-3, 3, n)]).astype(F32).reshape(3, ndet, nt)
+    lp = gmm.LocalPixelization.make_rectangle(ny, nx, periodic_xcoord=False)
+    pix = orc.Pix.rectangle(ny, nx, False)
+    xg = to_gpu(xp, dev)
+    plan = gmm.PointingPlan(gmm.PointingPrePlan(xg, ny, nx, periodic_xcoord=False), xg)
+    resp = rng.uniform(-2, 2, size=(2, 2 * ndet)).astype(F32)   # 420 samples per response detector
+    tod = rng.standard_normal((ndet, nt)).astype(F32)
+    t16, sc = gmm.encode_tod16(to_gpu(tod, dev), dtype)
+    dec = _decode(t16, sc)
+    out = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
+    gmm.tod2map(out, t16, xg, plan, tod_scale=sc, response=to_gpu(resp, dev))
+    ref, _ = orc.response_tod2map(dec, xp, resp, pix)
+    assert l1rel(to_cpu(out.arr), ref) < 1e-6
+    m = rng.standard_normal(3 * lp.npix).astype(F32)
+    o16 = torch.empty_like(t16)
+    gmm.map2tod(o16, gmm.LocalMap(lp, to_gpu(m, dev)), xg, plan, tod_scale=sc, response=to_gpu(resp, dev))
+    ref, _ = orc.response_map2tod(m, xp, resp, pix)
+    assert l1rel(_decode(o16, sc), ref) < eps
+
+
+@pytest.mark.parametrize("kind", ["f32", "f16"])
+def test_tod2map_nonfinite_fallback(gmm, orc, dev, kind):
+    """A NaN sample: the touched pixels become NaN and every other pixel stays exact."""
+    rng = np.random.default_rng(5)
+    ny, nx, n = 128, 256, 32 * 512
+    t = np.arange(n)
+    xp = np.stack([64 + 50 * np.sin(t / 500.0), 128 + 100 * np.sin(t / 700.0),
+                   rng.uniform(-3, 3, n)]).astype(F32)
+    lp = gmm.LocalPixelization.make_rectangle(ny, nx, periodic_xcoord=False)
+    pix = orc.Pix.rectangle(ny, nx, False)
+    xg = to_gpu(xp, dev)
+    plan = gmm.PointingPlan(gmm.PointingPrePlan(xg, ny, nx, periodic_xcoord=False), xg)
+    tod = rng.standard_normal(n).astype(F32)
+    tod[1234] = np.nan
+    out = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
+    if kind == "f32":
+        gmm.tod2map(out, to_gpu(tod, dev), xg, plan)
+        dec = tod
+    else:
+        t16, sc = gmm.encode_tod16(to_gpu(tod, dev), torch.float16)
+        gmm.tod2map(out, t16, xg, plan, tod_scale=sc)
+        dec = (t16.float() * sc).cpu().numpy()
+    ref, _ = orc.tod2map(dec.astype(np.float64), xp, pix, f64=True)
+    got = to_cpu(out.arr)
+    assert np.array_equal(np.isnan(got), np.isnan(ref))
+    fin = ~np.isnan(ref)
+    assert np.isnan(ref).sum() > 0
+    assert l1rel(got[fin], ref[fin]) < 1e-6

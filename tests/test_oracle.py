@@ -368,3 +368,72 @@ def test_lpix_rules(orc):
     assert orc.lpix_check(128, 128, bad, 64, 4096)[0] == 6
     gap = off.copy(); gap[1, 1] = 3 * 12288
     assert orc.lpix_check(128, 128, gap, 64, 4096)[0] == 7
+
+
+# ------------------------------------------------------------------------------------------
+# Response operators (per-detector T and P gains), pinned by decomposition into the plain
+# operators (themselves pinned above): with the samples split by detector, T and QU parts of
+# the map are scaled independently. Plus the reference's own check (gpu_mm/tests.py:295-315:
+# response (2, -1) with the map divided by (2, -1) leaves the TOD unchanged) and adjointness.
+
+def _resp_setup(orc, seed, periodic, ndet=6, nper=700):
+    rng = np.random.default_rng(seed)
+    ny, nx = 100, 190
+    pix = orc.Pix.rectangle(ny, nx, periodic)
+    xp = random_pointing(rng, ndet * nper, ny, nx, periodic)
+    resp = rng.uniform(-2, 2, size=(2, ndet)).astype(F32)
+    return rng, pix, xp, resp, ndet, nper
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_response_map2tod_decomposes(orc, periodic):
+    rng, pix, xp, resp, ndet, nper = _resp_setup(orc, 21, periodic)
+    m = rng.standard_normal(3 * pix.npix)
+    npix = pix.npix
+    m_t = m.copy(); m_t[npix:] = 0.0
+    m_p = m.copy(); m_p[:npix] = 0.0
+    tod_t, _ = orc.map2tod(m_t, xp, pix, f64=True)
+    tod_p, _ = orc.map2tod(m_p, xp, pix, f64=True)
+    det = np.arange(ndet * nper) // nper
+    expect = resp[0, det] * tod_t + resp[1, det] * tod_p
+    got, err = orc.response_map2tod(m, xp, resp, pix)
+    assert err == 0
+    np.testing.assert_allclose(got, expect, rtol=1e-12, atol=1e-12)
+    base = rng.standard_normal(ndet * nper)
+    got2, _ = orc.response_map2tod(m, xp, resp, pix, tod=base, accum=True)
+    np.testing.assert_allclose(got2, base + expect, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_response_tod2map_decomposes(orc, periodic):
+    rng, pix, xp, resp, ndet, nper = _resp_setup(orc, 22, periodic)
+    d = rng.standard_normal(ndet * nper)
+    npix = pix.npix
+    det = np.arange(ndet * nper) // nper
+    expect = np.zeros(3 * npix)
+    for i in range(ndet):
+        di = np.where(det == i, d, 0.0)
+        mi, _ = orc.tod2map(di, xp, pix, f64=True)
+        expect[:npix] += resp[0, i] * mi[:npix]
+        expect[npix:] += resp[1, i] * mi[npix:]
+    got, err = orc.response_tod2map(d, xp, resp, pix)
+    assert err == 0
+    np.testing.assert_allclose(got, expect, rtol=1e-10, atol=1e-10)
+
+
+def test_response_reference_check_and_adjoint(orc):
+    rng, pix, xp, _, ndet, nper = _resp_setup(orc, 23, True)
+    npix = pix.npix
+    m = rng.standard_normal(3 * pix.npix)
+    plain, _ = orc.map2tod(m, xp, pix, f64=True)
+    resp = np.zeros((2, ndet), dtype=F32)
+    resp[0], resp[1] = 2.0, -1.0
+    m2 = m.copy(); m2[:npix] /= 2.0; m2[npix:] /= -1.0
+    got, _ = orc.response_map2tod(m2, xp, resp, pix)
+    np.testing.assert_allclose(got, plain, rtol=1e-12, atol=1e-12)
+    resp = rng.uniform(-2, 2, size=(2, ndet)).astype(F32)
+    d = rng.standard_normal(ndet * nper)
+    am, _ = orc.response_map2tod(m, xp, resp, pix)
+    atd, _ = orc.response_tod2map(d, xp, resp, pix)
+    lhs, rhs = np.dot(am, d), np.dot(m, atd)
+    assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + np.linalg.norm(am) * np.linalg.norm(d))
