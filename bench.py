@@ -1,4 +1,4 @@
-"""Benchmark: planned FP32 map2tod + tod2map on one MI355X.
+"""Benchmark: planned map2tod + tod2map on one MI355X, plain FP32 and 16-bit TOD forms.
 
 Workload (single-TOD full scale): ndet=2048 x nt=262144 samples (5.4e8), constant-elevation
 raster x = 4096 + 3000 tri(t/40000) + dx, y = 500 + 0.01 t + dy (detector offsets uniform in a
@@ -7,9 +7,12 @@ sparse LocalPixelization of the 64x64 cells the scan touches; map values are a G
 field with k^-2 power per component, TOD values N(0,1). Synthetic data, generated on the GPU.
 
 A step is one planned map2tod plus one planned tod2map over the whole TOD (inputs resident in
-HBM). value = samples per second through the step, in Gsamp/s.
+HBM). Forms: plain FP32 TOD; FP16 and BF16 TODs with one FP32 scale per detector. Each form's
+map error (tod2map) and TOD error (map2tod) are measured against the FP64 oracle on the first
+--err-ndet detectors. The chosen form is the fastest one whose RMS map error is <= 1e-4 of the
+map RMS; value = its samples per second through a step, in Gsamp/s.
 
-Prints one JSON line (rank 0). Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Prints one JSON line. Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 
 import argparse
@@ -25,8 +28,10 @@ if REPO not in sys.path:
     sys.path.insert(0, REPO)
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
-BYTES_PER_SAMPLE = 16      # 12 B pointing + 4 B TOD
 CELL_BYTES = 3 * 64 * 64 * 4
+MAP_ERR_BAR = 1e-4
+# Algorithmic bytes per sample: 12 B pointing + the TOD element.
+BYTES_PER_SAMPLE = {"f32": 16, "f16": 14, "bf16": 14}
 
 
 def log(*a):
@@ -65,6 +70,90 @@ def build_workload(torch, gmm, synth, dev, ndet, nt):
     return dict(xp=xp, pre=pre, plan=plan, lp=lp, lmap=lmap, tod=tod, ncells=ncells, ny=ny, nx=nx)
 
 
+def form_buffers(torch, gmm, form, w, lmap, plan):
+    """Input TOD, output TOD and their scales for one form (untimed set-up)."""
+    if form == "f32":
+        return dict(tod=w["tod"], out=torch.empty_like(w["tod"]), sc_in=None, sc_out=None)
+    dt = {"f16": torch.float16, "bf16": torch.bfloat16}[form]
+    t16, sc = gmm.encode_tod16(w["tod"], dt)
+    # Output scales fitted to the map2tod output (the FP32 result, encoded once).
+    o32 = torch.empty_like(w["tod"])
+    gmm.map2tod(o32, lmap, w["xp"], plan)
+    o16, osc = gmm.encode_tod16(o32, dt)
+    del o32
+    return dict(tod=t16, out=o16, sc_in=sc, sc_out=osc)
+
+
+def run_form(torch, gmm, bufs, lmap, out_map, xp, plan, steps, warmup):
+    def step():
+        gmm.map2tod(bufs["out"], lmap, xp, plan, tod_scale=bufs["sc_out"])
+        gmm.tod2map(out_map, bufs["tod"], xp, plan, tod_scale=bufs["sc_in"])
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+
+    # Timed region: exactly K steps; kernel durations from HIP events recorded by the library
+    # on the launch stream around each planned_map2tod_kernel / planned_tod2map_kernel.
+    gmm.kernel_timer_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t0
+    m_ms, m_n = gmm.kernel_timer_read("map2tod")
+    t_ms, t_n = gmm.kernel_timer_read("tod2map")
+    gmm.kernel_timer_enable(False)
+    plan.check_errflags("bench")
+    assert m_n == steps and t_n == steps, (m_n, t_n)
+    return t_total, m_ms / m_n * 1e-3, t_ms / t_n * 1e-3
+
+
+def form_errors(torch, gmm, w, forms, ndet_err):
+    """Map and TOD errors of each form against the FP64 oracle, on the first ndet_err detectors."""
+    from oracle import pyoracle as orc
+    ny, nx, lp = w["ny"], w["nx"], w["lp"]
+    xs = w["xp"][:, :ndet_err].contiguous()
+    ts = w["tod"][:ndet_err].contiguous()
+    plan = gmm.PointingPlan(gmm.PointingPrePlan(xs, ny, nx, periodic_xcoord=False), xs)
+    lmap = gmm.LocalMap(lp, w["lmap"])
+    pix = orc.Pix(ny, nx, lp.cell_offsets_cpu, lp.ystride, lp.polstride, False)
+    xs_h = xs.cpu().numpy()
+    t0 = time.perf_counter()
+    m_ref, e1 = orc.tod2map(ts.cpu().numpy().astype(np.float64), xs_h, pix, f64=True)
+    t_ref, e2 = orc.map2tod(w["lmap"].cpu().numpy().astype(np.float64), xs_h, pix, f64=True)
+    assert e1 == 0 and e2 == 0
+    log(f"[bench] FP64 oracle on {ndet_err} detectors: {time.perf_counter() - t0:.1f}s")
+
+    npix = lp.npix
+    hit = m_ref[:npix] != 0.0
+    out = {}
+    for form in forms:
+        b = form_buffers(torch, gmm, form, dict(w, xp=xs, tod=ts), lmap, plan)
+        m = gmm.LocalMap(lp, torch.zeros_like(w["lmap"]))
+        gmm.tod2map(m, b["tod"], xs, plan, tod_scale=b["sc_in"])
+        gmm.map2tod(b["out"], lmap, xs, plan, tod_scale=b["sc_out"])
+        got_m = m.arr.cpu().numpy().astype(np.float64)
+        if form == "f32":
+            got_t = b["out"].cpu().numpy().astype(np.float64)
+        else:
+            got_t = (b["out"].float() * b["sc_out"][:, None]).cpu().numpy().astype(np.float64)
+        comp = {}
+        for c, name in enumerate("TQU"):
+            ref = m_ref[c * npix:(c + 1) * npix][hit]
+            d = got_m[c * npix:(c + 1) * npix][hit] - ref
+            rms = float(np.sqrt(np.mean(ref ** 2)))
+            comp[name] = dict(rms_rel=float(np.sqrt(np.mean(d ** 2))) / rms, max_rel=float(np.max(np.abs(d))) / rms)
+        tod_l1 = float(np.sum(np.abs(got_t.ravel() - t_ref)) / np.sum(np.abs(t_ref)))
+        out[form] = dict(map_rms_rel=max(v["rms_rel"] for v in comp.values()),
+                         map_max_rel=max(v["max_rel"] for v in comp.values()),
+                         map_by_component=comp, tod_l1_rel=tod_l1)
+        log(f"[bench] {form}: map error vs FP64 (RMS / max, relative to map RMS) "
+            f"{out[form]['map_rms_rel']:.2e} / {out[form]['map_max_rel']:.2e}; TOD L1-rel {tod_l1:.2e}")
+    return out
+
+
 def time_plan_construction(torch, gmm, w, reps=3):
     xp, pre = w["xp"], w["pre"]
     buf = torch.empty(pre.plan_nbytes, dtype=torch.uint8, device=xp.device)
@@ -84,7 +173,7 @@ def time_plan_construction(torch, gmm, w, reps=3):
     return 1e3 * float(np.median(t_pre)), 1e3 * float(np.median(t_plan))
 
 
-def cpu_baseline(w, ndet_sample, torch):
+def cpu_baseline(w, ndet_sample):
     """Oracle (the CPU restatement of reference_map2tod/tod2map) on the host cores, on the first
     ndet_sample detectors of the same workload."""
     from oracle import pyoracle as orc
@@ -124,6 +213,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--ndet", type=int, default=2048)
     ap.add_argument("--nt", type=int, default=262144)
+    ap.add_argument("--forms", default="f32,f16,bf16")
+    ap.add_argument("--err-ndet", type=int, default=64, help="detectors used for the FP64 error check (0: skip)")
     ap.add_argument("--cpu-ndet", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-ref", action="store_true",
@@ -140,67 +231,36 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda", 0)
+    forms = [f for f in args.forms.split(",") if f]
+    assert all(f in BYTES_PER_SAMPLE for f in forms) and "f32" in forms
     w = build_workload(torch, gmm, synth, dev, args.ndet, args.nt)
     nsamp = args.ndet * args.nt
+    ncells = w["ncells"]
     lmap = gmm.LocalMap(w["lp"], w["lmap"])
     out_map = gmm.LocalMap(w["lp"], torch.zeros_like(w["lmap"]))
-    tod_out = torch.empty_like(w["tod"])
-    plan, xp, tod = w["plan"], w["xp"], w["tod"]
-
+    plan, xp = w["plan"], w["xp"]
     gmm.set_errflag_check(False)
 
-    def step():
-        gmm.map2tod(tod_out, lmap, xp, plan)
-        gmm.tod2map(out_map, tod, xp, plan)
+    res = {}
+    for form in forms:
+        bufs = form_buffers(torch, gmm, form, w, lmap, plan)
+        t_total, t_m2t, t_t2m = run_form(torch, gmm, bufs, lmap, out_map, xp, plan, args.steps, args.warmup)
+        del bufs
+        bps = BYTES_PER_SAMPLE[form]
+        ops = {"map2tod": dict(seconds=t_m2t, bytes=bps * nsamp + CELL_BYTES * ncells),
+               "tod2map": dict(seconds=t_t2m, bytes=bps * nsamp + 2 * CELL_BYTES * ncells)}
+        for k, v in ops.items():
+            v["gsamp_s"] = nsamp / v["seconds"] / 1e9
+            v["gb_s"] = v["bytes"] / v["seconds"] / 1e9
+            v["frac_hbm"] = v["bytes"] / v["seconds"] / HBM_PEAK
+            log(f"[bench] {form} {k} kernel: {v['seconds'] * 1e3:.3f} ms  {v['gsamp_s']:.1f} Gsamp/s  "
+                f"{v['gb_s']:.0f} GB/s  ({100 * v['frac_hbm']:.1f}% of 8 TB/s)")
+        res[form] = dict(t_total=t_total, ops=ops, gsamp_s=nsamp / (t_total / args.steps) / 1e9)
+        log(f"[bench] {form} step: {1e3 * t_total / args.steps:.3f} ms  {res[form]['gsamp_s']:.1f} Gsamp/s")
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-
-    # Timed region: exactly K steps.
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    t_total = time.perf_counter() - t0
-    plan.check_errflags("bench")
-
-    # Per-operator kernel time with HIP events on the launch stream (torch's current stream).
-    stream = torch.cuda.current_stream()
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
-    for i in range(args.steps):
-        ev[i][0].record(stream)
-        gmm.map2tod(tod_out, lmap, xp, plan)
-        ev[i][1].record(stream)
-        ev[i][2].record(stream)
-        gmm.tod2map(out_map, tod, xp, plan)
-        ev[i][3].record(stream)
-    torch.cuda.synchronize()
-    plan.check_errflags("bench")
-    t_m2t = float(np.mean([e[0].elapsed_time(e[1]) for e in ev])) * 1e-3
-    t_t2m = float(np.mean([e[2].elapsed_time(e[3]) for e in ev])) * 1e-3
-
-    ncells = w["ncells"]
-    bytes_m2t = BYTES_PER_SAMPLE * nsamp + CELL_BYTES * ncells
-    bytes_t2m = BYTES_PER_SAMPLE * nsamp + 2 * CELL_BYTES * ncells
-    ops = {
-        "map2tod": dict(seconds=t_m2t, bytes=bytes_m2t),
-        "tod2map": dict(seconds=t_t2m, bytes=bytes_t2m),
-    }
-    for k, v in ops.items():
-        v["gsamp_s"] = nsamp / v["seconds"] / 1e9
-        v["gb_s"] = v["bytes"] / v["seconds"] / 1e9
-        v["frac_hbm"] = v["bytes"] / v["seconds"] / HBM_PEAK
-        log(f"[bench] {k}: {v['seconds'] * 1e3:.3f} ms  {v['gsamp_s']:.1f} Gsamp/s  "
-            f"{v['gb_s']:.0f} GB/s  ({100 * v['frac_hbm']:.1f}% of 8 TB/s)")
-    dom = max(ops, key=lambda k: ops[k]["seconds"])
-
-    traffic = None
-    tfile = os.path.join(REPO, "profiles", f"traffic_{dom}_cfg1.json")
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+    errs = form_errors(torch, gmm, w, forms, min(args.err_ndet, args.ndet)) if args.err_ndet > 0 else {}
+    ok = [f for f in forms if f == "f32" or (f in errs and errs[f]["map_rms_rel"] <= MAP_ERR_BAR)]
+    chosen = max(ok, key=lambda f: res[f]["gsamp_s"])
 
     if args.copy_ref:
         src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
@@ -213,41 +273,51 @@ def main():
     t_pre, t_plan = time_plan_construction(torch, gmm, w)
     log(f"[bench] PointingPrePlan {t_pre:.2f} ms, PointingPlan {t_plan:.2f} ms")
 
-    cpu = None if args.no_cpu else cpu_baseline(w, min(args.cpu_ndet, args.ndet), torch)
+    cpu = None if args.no_cpu else cpu_baseline(w, min(args.cpu_ndet, args.ndet))
 
-    ms_per_step = 1e3 * t_total / args.steps
-    value = nsamp / (t_total / args.steps) / 1e9
-    a = ops[dom]["bytes"] / ops[dom]["seconds"]
+    r = res[chosen]
+    dom = max(r["ops"], key=lambda k: r["ops"][k]["seconds"])
+    traffic = None
+    tfile = os.path.join(REPO, "profiles", f"traffic_{dom}_{chosen}_cfg1.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    a = r["ops"][dom]["bytes"] / r["ops"][dom]["seconds"]
     line = {
         "metric": "Gsamp/s map2tod+tod2map & %HBM peak, chosen vs plain FP32 at <=1e-4 rel err",
-        "value": round(value, 3),
+        "value": round(r["gsamp_s"], 3),
         "unit": "Gsamp/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(1e3 * r["t_total"] / args.steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (GPU-generated raster pointing, GRF map, N(0,1) TOD)",
         "config": {
-            "workload": "planned map2tod+tod2map, plain FP32, single TOD 2048x262144 on sparse 3x4096x8192 map",
+            "workload": "planned map2tod+tod2map, single TOD 2048x262144 on sparse 3x4096x8192 map",
             "ndet": args.ndet, "nt": args.nt, "nsamp": nsamp, "cells": ncells, "plan_nmt": int(plan.plan_nmt),
-            "chunks": int(plan.nchunks), "form": "plain FP32 (chosen form)",
+            "chunks": int(plan.nchunks), "chosen_form": chosen, "tod_storage": chosen,
+            "map_error_bar": MAP_ERR_BAR, "error_sample": f"first {args.err_ndet} detectors vs FP64 oracle",
         },
-        "ops": {k: {"ms": round(v["seconds"] * 1e3, 4), "gsamp_s": round(v["gsamp_s"], 2),
-                    "gb_s": round(v["gb_s"], 1), "frac_hbm": round(v["frac_hbm"], 4)} for k, v in ops.items()},
+        "forms": {f: {"gsamp_s": round(v["gsamp_s"], 2), "ms_per_step": round(1e3 * v["t_total"] / args.steps, 4),
+                      "vs_f32": round(v["gsamp_s"] / res["f32"]["gsamp_s"], 3),
+                      "ops": {k: {"kernel_ms": round(o["seconds"] * 1e3, 4), "gsamp_s": round(o["gsamp_s"], 2),
+                                  "gb_s": round(o["gb_s"], 1), "frac_hbm": round(o["frac_hbm"], 4)}
+                              for k, o in v["ops"].items()},
+                      "errors": errs.get(f)} for f, v in res.items()},
         "plan_ms": {"preplan": round(t_pre, 3), "plan": round(t_plan, 3)},
         "roofline": {
-            "kernel": f"planned_{dom}_kernel",
+            "kernel": f"planned_{dom}_kernel ({chosen} TOD)",
             "bound": "hbm",
             "achieved": round(a / 1e9, 1),
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": round(a / HBM_PEAK, 4),
             "traffic": traffic,
-            "algorithmic_bytes_per_launch": int(ops[dom]["bytes"]),
+            "algorithmic_bytes_per_launch": int(r["ops"][dom]["bytes"]),
         },
         "cpu_baseline": cpu,
     }

@@ -38,6 +38,7 @@ LIB_SOURCES = [
     ("k_planned.hip", []),
     ("k_unplanned.hip", []),
     ("k_tod16.hip", []),
+    ("k_dynmap.hip", []),
 ]
 
 LIB_NAME = "libgpu_mm_amd.so"

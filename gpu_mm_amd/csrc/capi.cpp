@@ -685,6 +685,67 @@ int gmm_response_tod2map(float *local_map_gpu, int64_t local_map_size,
 }
 
 namespace {
+void expand_dynamic(uint32_t *global_ncells_gpu, int64_t *cell_offsets_gpu, int64_t nycells, int64_t nxcells,
+                    const uint64_t *plan_mt_gpu, int64_t nmt, uint32_t *new_ncells, hipStream_t stream)
+{
+    void *scratch = nullptr;
+    const int64_t nb = expand_dynamic_scratch_nbytes(nmt);
+    GMM_HIP(hipMalloc(&scratch, size_t(nb)));
+    try {
+        launch_expand_dynamic(global_ncells_gpu, cell_offsets_gpu, nycells, nxcells, plan_mt_gpu, nmt, scratch, stream);
+        uint32_t n = 0;
+        GMM_HIP(hipMemcpyAsync(&n, global_ncells_gpu, 4, hipMemcpyDeviceToHost, stream));
+        GMM_HIP(hipStreamSynchronize(stream));
+        if (new_ncells)
+            *new_ncells = n;
+    }
+    catch (...) {
+        (void) hipFree(scratch);
+        throw;
+    }
+    GMM_HIP(hipFree(scratch));
+}
+}  // namespace
+
+int gmm_expand_dynamic_map(uint32_t *global_ncells_gpu, int64_t *cell_offsets_gpu, int64_t nycells, int64_t nxcells,
+                           const uint64_t *plan_mt_gpu, int64_t nmt, uint32_t *new_ncells, void *stream_)
+{
+    return guarded([&] {
+        const char *where = "expand_dynamic_map";
+        require_ptr(global_ncells_gpu, where, "global_ncells");
+        require_ptr(cell_offsets_gpu, where, "cell_offsets");
+        if (nycells <= 0 || nycells > 1024 || nxcells <= 0 || nxcells > 1024)
+            fail(std::string(where) + ": expected 0 < nycells, nxcells <= 1024");
+        if (nmt < 0 || nmt > int64_t(UINT32_MAX) - 1)
+            fail(std::string(where) + ": bad nmt");
+        if (nmt > 0)
+            require_ptr(plan_mt_gpu, where, "plan_mt");
+        expand_dynamic(global_ncells_gpu, cell_offsets_gpu, nycells, nxcells, plan_mt_gpu, nmt, new_ncells,
+                       as_stream(stream_));
+    });
+}
+
+int gmm_lpix_expand_dynamic(gmm_lpix *lpix, const gmm_plan *plan, uint32_t *global_ncells_gpu, uint32_t *new_ncells,
+                            void *stream_)
+{
+    return guarded([&] {
+        const char *where = "expand_dynamic_map2";
+        require_ptr(lpix, where, "lpix");
+        require_ptr(plan, where, "plan");
+        require_ptr(global_ncells_gpu, where, "global_ncells");
+        require_ptr(lpix->offsets_gpu, where, "cell_offsets_gpu");
+        const gmm_preplan &pp = plan->pp;
+        if (lpix->nypix_global != pp.nypix_global || lpix->nxpix_global != pp.nxpix_global
+            || lpix->periodic_xcoord != pp.periodic_xcoord)
+            fail(std::string(where) + ": LocalPixelization and PointingPlan disagree on (nypix_global, nxpix_global, periodic_xcoord)");
+        if (lpix->nycells != (pp.nypix_global + 63) / 64 || lpix->nxcells != (pp.nxpix_global + 63) / 64)
+            fail(std::string(where) + ": a dynamic map needs the full (nycells, nxcells) cell table");
+        expand_dynamic(global_ncells_gpu, const_cast<int64_t *>(lpix->offsets_gpu), lpix->nycells, lpix->nxcells,
+                       plan->plan_mt, pp.plan_nmt, new_ncells, as_stream(stream_));
+    });
+}
+
+namespace {
 // Arguments of the 16-bit TOD entry points.
 TodGains q16_gains(int tod_type, const void *tod, const float *scale, int64_t nrows, int64_t nsamp,
                    const float *response, int64_t resp_ndet, const char *where)

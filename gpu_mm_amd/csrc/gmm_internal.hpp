@@ -166,6 +166,12 @@ void launch_planned_map2tod(void *tod, const float *lmap, const float *xp, const
                             const gmm_plan &plan, bool accum, const TodGains &g, hipStream_t stream);
 void launch_planned_tod2map(float *lmap, const void *tod, const float *xp, const OpArgs &a,
                             const gmm_plan &plan, const TodGains &g, hipStream_t stream);
+// Dynamic map expansion (cells of a cell-sorted plan_mt into cell_offsets, new indices in
+// cell-key order); scratch of expand_dynamic_scratch_nbytes(nmt) bytes.
+int64_t expand_dynamic_scratch_nbytes(int64_t nmt);
+void launch_expand_dynamic(uint32_t *global_ncells, int64_t *cell_offsets, int64_t nycells, int64_t nxcells,
+                           const uint64_t *plan_mt, int64_t nmt, void *scratch, hipStream_t stream);
+
 // Per-row 16-bit encoding of a float32 TOD [nrows][row_len]: scale[r] = max(max_s |tod[r][s]| /
 // (0.5 * largest finite value of the type), FLT_MIN), out[r][s] = tod[r][s] / scale[r] rounded
 // to nearest even.

@@ -582,6 +582,37 @@ PYBIND11_MODULE(_gpu_mm_amd_py, m)
     m.def("planned_tod2map_q16", &planned_tod2map_q16, py::arg("local_map"), py::arg("tod"), py::arg("tod_scale"),
           py::arg("xpointing"), py::arg("local_pixelization"), py::arg("plan"), py::arg("partial_pixelization"),
           py::arg("debug"), py::arg("response") = py::none(), py::arg("check_errors") = true, py::arg("stream") = 0);
+    m.def("expand_dynamic_map", [](py::object global_ncells, py::object cell_offsets, py::object plan_mt, uintptr_t stream) {
+        const char *where = "expand_dynamic_map";
+        Arr g = get_arr(global_ncells, where, "global_ncells");
+        require(g, where, "global_ncells", kDLUInt, 32, true, "uint32");
+        Arr c = get_arr(cell_offsets, where, "cell_offsets");
+        require(c, where, "cell_offsets", kDLInt, 64, true, "int64");
+        Arr m = get_arr(plan_mt, where, "plan_mt");
+        require(m, where, "plan_mt", kDLUInt, 64, true, "uint64");
+        if (g.size != 1 || c.shape.size() != 2 || m.shape.size() != 1)
+            raise(std::string(where) + ": expected global_ncells of shape (1,), cell_offsets 2-d, plan_mt 1-d");
+        uint32_t n = 0;
+        {
+            py::gil_scoped_release nogil;
+            check(gmm_expand_dynamic_map(static_cast<uint32_t *>(g.ptr), static_cast<int64_t *>(c.ptr), c.shape[0],
+                                         c.shape[1], static_cast<const uint64_t *>(m.ptr), m.size, &n, as_stream(stream)));
+        }
+        return n;
+    }, py::arg("global_ncells"), py::arg("cell_offsets"), py::arg("plan_mt"), py::arg("stream") = 0,
+       "Adds the targeted (-1) cells named by plan_mt to cell_offsets; returns the new cell count");
+    m.def("expand_dynamic_map2", [](py::object global_ncells, LocalPixelization &lpix, const PointingPlan &plan,
+                                    uintptr_t stream) {
+        const char *where = "expand_dynamic_map2";
+        Arr g = get_arr(global_ncells, where, "global_ncells");
+        require(g, where, "global_ncells", kDLUInt, 32, true, "uint32");
+        uint32_t n = 0;
+        {
+            py::gil_scoped_release nogil;
+            check(gmm_lpix_expand_dynamic(lpix.h, plan.h, static_cast<uint32_t *>(g.ptr), &n, as_stream(stream)));
+        }
+        return n;
+    }, py::arg("global_ncells"), py::arg("local_pixelization"), py::arg("plan"), py::arg("stream") = 0);
     m.def("kernel_timer_enable", [](bool on) { check(gmm_kernel_timer_enable(on ? 1 : 0)); });
     m.def("kernel_timer_read", [](int which) {
         double ms = 0.0;

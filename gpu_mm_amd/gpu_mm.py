@@ -94,6 +94,16 @@ def encode_tod16(tod, dtype=torch.float16):
     return out, scale
 
 
+def kernel_timer_enable(enable=True):
+    """Starts (or stops) recording HIP events around every planned map2tod/tod2map kernel."""
+    _ext.kernel_timer_enable(bool(enable))
+
+
+def kernel_timer_read(kernel):
+    """(total_ms, launches) of 'map2tod' or 'tod2map' kernels since the last enable/read."""
+    return _ext.kernel_timer_read({"map2tod": 0, "tod2map": 1}[kernel])
+
+
 def map2tod(tod, local_map, xpointing, plan, partial_pixelization=False, response=None, debug=False, accum=False,
             tod_scale=None):
     """
@@ -153,10 +163,22 @@ def tod2map(local_map, tod, xpointing, plan, partial_pixelization=False, respons
     accumulated into.
     Reference: gpu_mm/gpu_mm.py:314-385.
     """
-    if not isinstance(local_map, LocalMap):
-        raise RuntimeError(f"tod2map(): Bad 'local_map' argument to tod2map(): expected LocalMap, got: {local_map}")
-    lpix = local_map.pixelization
-    larr = local_map.arr
+    if isinstance(local_map, DynamicMap):
+        if not isinstance(plan, PointingPlan):
+            raise RuntimeError("tod2map(): if output is a DynamicMap, then the 'plan' argument"
+                               + f" must be a PointingPlan (got: {plan})'")
+        if partial_pixelization != local_map.have_cell_mask:
+            raise RuntimeError(f"tod2map(): {partial_pixelization=} was specified,"
+                               + f" but DynamicMap.have_cell_mask={local_map.have_cell_mask}"
+                               + " (expected these to be equal)")
+        local_map.expand(plan)
+        lpix = local_map._unstable_pixelization
+        larr = local_map._unstable_arr
+    elif isinstance(local_map, LocalMap):
+        lpix = local_map.pixelization
+        larr = local_map.arr
+    else:
+        raise RuntimeError(f"tod2map(): Bad 'local_map' argument to tod2map(): expected LocalMap or DynamicMap, got: {local_map}")
     if (response is not None) and not isinstance(plan, PointingPlan):
         raise RuntimeError("tod2map(): 'response' requires a PointingPlan")
 
@@ -330,6 +352,145 @@ class LocalMap:
                     idx = o + pol * p.polstride + iy * p.ystride + ix
                     dst[pol, y0:y0 + hy, x0:x0 + hx] = src[idx][:hy, :hx]
         return dst
+
+
+####################################################################################################
+
+
+class DynamicMap:
+    """
+    A local map that starts with no cells and gains cells as tod2map() needs them; finalize()
+    turns it into a LocalMap. Layout: float32 local_map[ncells][3][64][64] (ystride 64,
+    polstride 4096). cell_mask (bool, shape (nycells, nxcells)) restricts the cells that may be
+    added ("targeted" cells); tod2map() then needs partial_pixelization=True.
+
+    New cells get indices in increasing cell-key order within each expansion, so the layout is
+    reproducible (the reference's order is nondeterministic). Only tod2map() accepts a
+    DynamicMap, as in the reference.
+
+    Reference: gpu_mm/gpu_mm.py:586-806; src_lib/expand_dynamic_map.cu.
+    """
+
+    def __init__(self, nypix_global, nxpix_global, dtype, cell_mask=None, periodic_xcoord=True, initial_ncells=1024):
+        assert 0 < nxpix_global <= 64 * 1024
+        assert 0 < nypix_global <= 64 * 1024
+        assert dtype in (np.float32, torch.float32, 'float32'), "DynamicMap: only float32 is supported"
+        assert initial_ncells > 0
+
+        self.nypix_global = int(nypix_global)
+        self.nxpix_global = int(nxpix_global)
+        self.periodic_xcoord = bool(periodic_xcoord)
+        self.have_cell_mask = cell_mask is not None
+        self.ncells_allocated = int(initial_ncells)
+        self.reallocation_factor = 1.5
+        self.dtype = np.float32
+
+        nycells = (self.nypix_global + 63) // 64
+        nxcells = (self.nxpix_global + 63) // 64
+        if cell_mask is None:
+            cell_offsets = np.full((nycells, nxcells), -1, dtype=np.int64)
+        else:
+            cell_mask = np.asarray(cell_mask)
+            assert cell_mask.ndim == 2 and cell_mask.dtype == bool
+            assert cell_mask.shape == (nycells, nxcells)
+            cell_offsets = np.where(cell_mask, -1, -2).astype(np.int64)
+
+        # Between calls the CPU copy of the offsets is stale (the GPU copy is authoritative).
+        self._unstable_pixelization = LocalPixelization(self.nypix_global, self.nxpix_global, cell_offsets,
+                                                        ystride=64, polstride=64 * 64,
+                                                        periodic_xcoord=self.periodic_xcoord)
+        self._padded_arr = torch.zeros(self.ncells_allocated * 3 * 64 * 64, dtype=torch.float32, device=_device())
+        self._unstable_arr = self._padded_arr[:0]
+        self.global_ncells = torch.zeros(1, dtype=torch.uint32, device=_device())
+        self.ncells_curr = 0
+        self.pixelization_is_finalized = False
+        self.map_is_finalized = False
+
+    def expand(self, plan):
+        """Adds the cells a plan touches (zero-filled), reallocating the array when needed."""
+        if self.pixelization_is_finalized:
+            raise RuntimeError("Can't currently call tod2map() [or DynamicMap.expand()]"
+                               + " after DynamicMap.permute() or DynamicMap.finalize().")
+        assert isinstance(plan, PointingPlan)
+        assert plan.nypix_global == self.nypix_global
+        assert plan.nxpix_global == self.nxpix_global
+        assert plan.periodic_xcoord == self.periodic_xcoord
+
+        self.ncells_curr = int(_ext.expand_dynamic_map2(self.global_ncells, self._unstable_pixelization, plan,
+                                                         stream=_stream()))
+        if self.ncells_curr > self.ncells_allocated:
+            self.ncells_allocated = max(int(self.reallocation_factor * self.ncells_allocated), self.ncells_curr)
+            new = torch.zeros(self.ncells_allocated * 3 * 64 * 64, dtype=torch.float32, device=_device())
+            new[:self._unstable_arr.numel()] = self._unstable_arr
+            self._padded_arr = new
+        self._unstable_pixelization.npix = self.ncells_curr * 64 * 64
+        self._unstable_arr = self._padded_arr[:self.ncells_curr * 3 * 64 * 64]
+
+    def permute(self, iycells, ixcells):
+        """
+        Reorders the cells: cell i of the new layout is (iycells[i], ixcells[i]); the arrays (int,
+        length ncells_curr, on the CPU) must list exactly the present cells. Finalizes the
+        pixelization (no more expansion).
+        """
+        if self.pixelization_is_finalized:
+            raise RuntimeError("Can't call DynamicMap.permute() after DynamicMap.permute() or DynamicMap.finalize()")
+        lp = self._unstable_pixelization
+        lp.copy_gpu_offsets_to_cpu()
+        assert lp.npix == self.ncells_curr * 64 * 64
+        old = np.copy(lp.cell_offsets_cpu)
+        iycells, ixcells = np.asarray(iycells), np.asarray(ixcells)
+        assert iycells.shape == ixcells.shape == (self.ncells_curr,)
+        assert np.issubdtype(iycells.dtype, np.integer) and np.issubdtype(ixcells.dtype, np.integer)
+        if self.ncells_curr > 0:
+            assert 0 <= iycells.min() and iycells.max() < old.shape[0]
+            assert 0 <= ixcells.min() and ixcells.max() < old.shape[1]
+        new = np.copy(old)
+        new[iycells, ixcells] = np.arange(self.ncells_curr, dtype=np.int64) * (3 * 64 * 64)
+        if not np.array_equal(old >= 0, new >= 0) or len(set(zip(iycells.tolist(), ixcells.tolist()))) != self.ncells_curr:
+            raise RuntimeError("DynamicMap.permute(): specified (iycells,ixcells) do not match footprint of current pixelization")
+
+        src = torch.from_numpy(old[iycells, ixcells] // (3 * 64 * 64)).to(_device())
+        arr = self._unstable_arr.reshape(self.ncells_curr, 3 * 64 * 64)[src].reshape(-1).contiguous()
+        lp.cell_offsets_cpu[:, :] = new
+        lp.copy_cpu_offsets_to_gpu()
+        self.pixelization_is_finalized = True
+        self._padded_arr = self._unstable_arr = arr
+        self.ncells_allocated = self.ncells_curr
+
+    def finalize(self):
+        """Ends the TOD loop; returns the LocalMap."""
+        if self.map_is_finalized:
+            raise RuntimeError("Double call to DynamicMap.finalize() [not currently allowed]")
+        lp = self._unstable_pixelization
+        if not self.pixelization_is_finalized:
+            lp.copy_gpu_offsets_to_cpu()
+            self.pixelization_is_finalized = True
+        arr = self._unstable_arr
+        if self.ncells_allocated > self.ncells_curr:
+            arr = arr.clone()
+        assert lp.npix == self.ncells_curr * 64 * 64
+        ret = LocalMap(lp, arr)
+        self._padded_arr = self._unstable_arr = None
+        self.map_is_finalized = True
+        return ret
+
+    def randomly_permute(self, rng=None):
+        """Test aid: a random cell order."""
+        rng = np.random.default_rng() if rng is None else rng
+        self._unstable_pixelization.copy_gpu_offsets_to_cpu()
+        iy, ix = np.where(self._unstable_pixelization.cell_offsets_cpu >= 0)
+        perm = rng.permutation(self.ncells_curr)
+        self.permute(iy[perm], ix[perm])
+
+
+def expand_dynamic_map(global_ncells, cell_offsets, plan_mt):
+    """
+    Low-level expansion over a cell-sorted plan_mt array (uint64, GPU): every targeted (-1)
+    entry of cell_offsets (int64 (nycells, nxcells), GPU) named by plan_mt gets the next index.
+    Returns the new cell count (also left in global_ncells, uint32 (1,) on the GPU).
+    Reference: src_lib/expand_dynamic_map.cu:107-141.
+    """
+    return int(_ext.expand_dynamic_map(global_ncells, cell_offsets, plan_mt, stream=_stream()))
 
 
 ####################################################################################################

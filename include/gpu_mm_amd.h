@@ -175,6 +175,22 @@ int gmm_response_tod2map(float *local_map_gpu, int64_t local_map_size,
                          const gmm_lpix *lpix, const gmm_plan *plan,
                          int partial_pixelization, int debug, int check_errors, void *stream);
 
+/* ---- Dynamic maps --------------------------------------------------------------------------
+ * expand_dynamic_map, include/gpu_mm.hpp:268-276; src_lib/expand_dynamic_map.cu:107-141.
+ * cell_offsets_gpu: int64[nycells][nxcells] with -1 = targeted cell not yet present, -2 =
+ * untargeted, 3*64*64*i = the i-th cell present. Every targeted absent cell named by plan_mt
+ * (device, sorted by its low 20 bits) becomes present, with indices global_ncells, +1, ... in
+ * increasing cell-key order (the reference's order is nondeterministic). *global_ncells_gpu is
+ * advanced; the new count is also returned in *new_ncells (host). Synchronizes the stream. */
+int gmm_expand_dynamic_map(uint32_t *global_ncells_gpu, int64_t *cell_offsets_gpu, int64_t nycells, int64_t nxcells,
+                           const uint64_t *plan_mt_gpu, int64_t nmt, uint32_t *new_ncells, void *stream);
+
+/* expand_dynamic_map2, include/gpu_mm.hpp:278-283; src_lib/expand_dynamic_map.cu:144-160: the
+ * same over a plan's entries, into the LocalPixelization's GPU cell offsets (the full
+ * (nycells, nxcells) table). The caller updates npix and the CPU copy, as in the reference. */
+int gmm_lpix_expand_dynamic(gmm_lpix *lpix, const gmm_plan *plan, uint32_t *global_ncells_gpu, uint32_t *new_ncells,
+                            void *stream);
+
 /* ---- Kernel timer -------------------------------------------------------------------------
  * No counterpart in the reference; a measurement aid. While enabled, every planned operator
  * records a HIP event pair on its launch stream around its main kernel (planned_map2tod_kernel

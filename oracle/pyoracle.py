@@ -78,6 +78,10 @@ class Pix:
         self.cell_offsets = np.ascontiguousarray(cell_offsets, dtype=np.int64)
         self.ystride, self.polstride = int(ystride), int(polstride)
         self.npix = 4096 * int(np.sum(self.cell_offsets >= 0))
+        present = self.cell_offsets[self.cell_offsets >= 0]
+        if present.size and int(present.max()) + 2 * self.polstride + 63 * self.ystride + 63 >= 3 * self.npix:
+            # The C functions index the map through these offsets without bounds checks.
+            raise ValueError("Pix: cell offsets reach beyond the 3*npix map")
 
     @classmethod
     def rectangle(cls, ny, nx, periodic):
