@@ -39,6 +39,7 @@ LIB_SOURCES = [
     ("k_unplanned.hip", []),
     ("k_tod16.hip", []),
     ("k_dynmap.hip", []),
+    ("k_cells.hip", []),
 ]
 
 LIB_NAME = "libgpu_mm_amd.so"

@@ -210,6 +210,79 @@ int bits_for(int64_t n)
 
 using namespace gmm;
 
+namespace {
+// check_response(), src_lib/check_arguments.cu:171-180: response is float[2][ndet] (row 0: T
+// response, row 1: polarization response) and ndet must divide nsamp.
+void check_response(const float *response_gpu, int64_t ndet, int64_t nsamp, const char *where)
+{
+    require_ptr(response_gpu, where, "response");
+    if (ndet <= 0)
+        fail(std::string(where) + ": response array must have shape (2, ndet) with ndet > 0");
+    if (nsamp % ndet != 0)
+        fail(std::string(where) + ": response array has ndet=" + std::to_string(ndet)
+             + ", which does not divide nsamp=" + std::to_string(nsamp));
+    if (nsamp / ndet > int64_t(UINT32_MAX))
+        fail(std::string(where) + ": samples per detector must be < 2^32");
+}
+}  // namespace
+
+namespace {
+// Arguments of the 16-bit TOD entry points.
+TodGains q16_gains(int tod_type, const void *tod, const float *scale, int64_t nrows, int64_t nsamp,
+                   const float *response, int64_t resp_ndet, const char *where)
+{
+    if (tod_type != GMM_TOD_F16 && tod_type != GMM_TOD_BF16)
+        fail(std::string(where) + ": tod_type must be GMM_TOD_F16 or GMM_TOD_BF16");
+    if (reinterpret_cast<uintptr_t>(tod) & 7)
+        fail(std::string(where) + ": a 16-bit TOD must be 8-byte aligned");
+    require_ptr(scale, where, "tod_scale");
+    if (nrows <= 0 || nsamp % nrows != 0)
+        fail(std::string(where) + ": tod_scale has " + std::to_string(nrows)
+             + " rows, which does not divide nsamp=" + std::to_string(nsamp));
+    TodGains g;
+    g.tod_type = (tod_type == GMM_TOD_F16) ? TOD_F16 : TOD_BF16;
+    g.scale = scale;
+    g.scale_nrows = nrows;
+    if (response) {
+        check_response(response, resp_ndet, nsamp, where);
+        g.response = response;
+        g.resp_ndet = resp_ndet;
+    }
+    return g;
+}
+}  // namespace
+
+namespace {
+void check_cells(const float *p, const char *where, const char *name)
+{
+    require_ptr(p, where, name);
+    if (reinterpret_cast<uintptr_t>(p) & 15)
+        fail(std::string(where) + ": '" + name + "' must be 16-byte aligned");
+}
+
+// Runs a cell kernel with a device 'bad index' word, then checks it (synchronizes the stream).
+template<typename F>
+void with_bad_word(const char *where, hipStream_t stream, F f)
+{
+    uint32_t *bad = nullptr;
+    GMM_HIP(hipMalloc(reinterpret_cast<void **>(&bad), 4));
+    uint32_t h = 0;
+    try {
+        launch_fill_u32(bad, 1, 0u, stream);
+        f(bad);
+        GMM_HIP(hipMemcpyAsync(&h, bad, 4, hipMemcpyDeviceToHost, stream));
+        GMM_HIP(hipStreamSynchronize(stream));
+    }
+    catch (...) {
+        (void) hipFree(bad);
+        throw;
+    }
+    GMM_HIP(hipFree(bad));
+    if (h)
+        fail(std::string(where) + ": index_map entry out of range");
+}
+}  // namespace
+
 extern "C" {
 
 const char *gmm_last_error(void) { return g_last_error.c_str(); }
@@ -613,21 +686,6 @@ int gmm_planned_tod2map(float *local_map_gpu, int64_t local_map_size,
     });
 }
 
-namespace {
-// check_response(), src_lib/check_arguments.cu:171-180: response is float[2][ndet] (row 0: T
-// response, row 1: polarization response) and ndet must divide nsamp.
-void check_response(const float *response_gpu, int64_t ndet, int64_t nsamp, const char *where)
-{
-    require_ptr(response_gpu, where, "response");
-    if (ndet <= 0)
-        fail(std::string(where) + ": response array must have shape (2, ndet) with ndet > 0");
-    if (nsamp % ndet != 0)
-        fail(std::string(where) + ": response array has ndet=" + std::to_string(ndet)
-             + ", which does not divide nsamp=" + std::to_string(nsamp));
-    if (nsamp / ndet > int64_t(UINT32_MAX))
-        fail(std::string(where) + ": samples per detector must be < 2^32");
-}
-}  // namespace
 
 int gmm_response_map2tod(float *tod_gpu, int64_t tod_nsamp,
                          const float *local_map_gpu, int64_t local_map_size,
@@ -745,31 +803,6 @@ int gmm_lpix_expand_dynamic(gmm_lpix *lpix, const gmm_plan *plan, uint32_t *glob
     });
 }
 
-namespace {
-// Arguments of the 16-bit TOD entry points.
-TodGains q16_gains(int tod_type, const void *tod, const float *scale, int64_t nrows, int64_t nsamp,
-                   const float *response, int64_t resp_ndet, const char *where)
-{
-    if (tod_type != GMM_TOD_F16 && tod_type != GMM_TOD_BF16)
-        fail(std::string(where) + ": tod_type must be GMM_TOD_F16 or GMM_TOD_BF16");
-    if (reinterpret_cast<uintptr_t>(tod) & 7)
-        fail(std::string(where) + ": a 16-bit TOD must be 8-byte aligned");
-    require_ptr(scale, where, "tod_scale");
-    if (nrows <= 0 || nsamp % nrows != 0)
-        fail(std::string(where) + ": tod_scale has " + std::to_string(nrows)
-             + " rows, which does not divide nsamp=" + std::to_string(nsamp));
-    TodGains g;
-    g.tod_type = (tod_type == GMM_TOD_F16) ? TOD_F16 : TOD_BF16;
-    g.scale = scale;
-    g.scale_nrows = nrows;
-    if (response) {
-        check_response(response, resp_ndet, nsamp, where);
-        g.response = response;
-        g.resp_ndet = resp_ndet;
-    }
-    return g;
-}
-}  // namespace
 
 int gmm_planned_map2tod_q16(void *tod_gpu, int tod_type, const float *tod_scale_gpu, int64_t tod_nrows,
                             int64_t tod_nsamp, const float *local_map_gpu, int64_t local_map_size,
@@ -835,6 +868,63 @@ int gmm_tod16_encode(void *out_gpu, int tod_type, float *scale_gpu, const float 
             return;
         launch_tod16_encode(out_gpu, tod_type == GMM_TOD_F16 ? TOD_F16 : TOD_BF16, scale_gpu, tod_gpu, nrows,
                             row_len, as_stream(stream_));
+    });
+}
+
+
+int gmm_cell_broadcast(float *dst_gpu, int64_t ncells, const float *src_gpu, int64_t nsrc_cells,
+                       const int64_t *index_map_gpu, void *stream_)
+{
+    return guarded([&] {
+        const char *where = "cell_broadcast";
+        if (ncells < 0 || nsrc_cells < 0 || ncells > INT32_MAX)
+            fail(std::string(where) + ": bad cell counts");
+        if (ncells == 0)
+            return;
+        check_cells(dst_gpu, where, "dst");
+        check_cells(src_gpu, where, "src");
+        require_ptr(index_map_gpu, where, "index_map");
+        hipStream_t stream = as_stream(stream_);
+        with_bad_word(where, stream, [&](uint32_t *bad) {
+            launch_cell_broadcast(dst_gpu, ncells, src_gpu, nsrc_cells, index_map_gpu, bad, stream);
+        });
+    });
+}
+
+int gmm_cell_reduce(float *dst_gpu, int64_t ndst_cells, const float *src_gpu, int64_t nsrc_cells,
+                    const int64_t *index_map_gpu, void *stream_)
+{
+    return guarded([&] {
+        const char *where = "cell_reduce";
+        if (ndst_cells < 0 || nsrc_cells < 0 || ndst_cells > INT32_MAX)
+            fail(std::string(where) + ": bad cell counts");
+        if (ndst_cells == 0)
+            return;
+        check_cells(dst_gpu, where, "dst");
+        if (nsrc_cells > 0) {
+            check_cells(src_gpu, where, "src");
+            require_ptr(index_map_gpu, where, "index_map");
+        }
+        hipStream_t stream = as_stream(stream_);
+        with_bad_word(where, stream, [&](uint32_t *bad) {
+            launch_cell_reduce(dst_gpu, ndst_cells, src_gpu, nsrc_cells, index_map_gpu, bad, stream);
+        });
+    });
+}
+
+int gmm_local_map_to_global(const gmm_lpix *lpix, float *dst_gpu, const float *src_gpu, int64_t src_size,
+                            void *stream_)
+{
+    return guarded([&] {
+        const char *where = "local_map_to_global";
+        require_ptr(lpix, where, "lpix");
+        require_ptr(dst_gpu, where, "dst");
+        require_ptr(src_gpu, where, "src");
+        require_ptr(lpix->offsets_gpu, where, "cell_offsets_gpu");
+        if (src_size != 3 * lpix->npix)
+            fail(std::string(where) + ": expected local map of size 3*npix=" + std::to_string(3 * lpix->npix)
+                 + ", got " + std::to_string(src_size));
+        launch_local_to_global(dst_gpu, src_gpu, *lpix, as_stream(stream_));
     });
 }
 

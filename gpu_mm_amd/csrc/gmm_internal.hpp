@@ -172,6 +172,13 @@ int64_t expand_dynamic_scratch_nbytes(int64_t nmt);
 void launch_expand_dynamic(uint32_t *global_ncells, int64_t *cell_offsets, int64_t nycells, int64_t nxcells,
                            const uint64_t *plan_mt, int64_t nmt, void *scratch, hipStream_t stream);
 
+// Cell utilities (k_cells.hip). 'bad' (device word) is OR-ed with 1 on an out-of-range index.
+void launch_cell_broadcast(float *dst, int64_t ncells, const float *src, int64_t nsrc, const int64_t *index_map,
+                           uint32_t *bad, hipStream_t stream);
+void launch_cell_reduce(float *dst, int64_t ndst, const float *src, int64_t nsrc, const int64_t *index_map,
+                        uint32_t *bad, hipStream_t stream);
+void launch_local_to_global(float *dst, const float *src, const gmm_lpix &lp, hipStream_t stream);
+
 // Per-row 16-bit encoding of a float32 TOD [nrows][row_len]: scale[r] = max(max_s |tod[r][s]| /
 // (0.5 * largest finite value of the type), FLT_MIN), out[r][s] = tod[r][s] / scale[r] rounded
 // to nearest even.

@@ -1,25 +1,28 @@
 // Planned map2tod / tod2map on gfx950.
 //
-// One block per work unit ("chunk": a run of plan entries of one cell, cut at plan time). The
-// block stages the cell's 3x64x64 T/Q/U values in LDS (map2tod) or accumulates them there
-// (tod2map). Each wave takes a contiguous slice of the chunk's entries and caches 64 plan
-// entries per vector load. A wave step covers 8 entries (8 TOD lines of 32 samples): lane
-// 8*li + q owns samples 4q..4q+3 of line li, loaded as float4, and two such steps are loaded
-// before either is used.
+// One block per work unit ("chunk": a run of plan entries of one cell, cut at plan time). Each
+// wave takes a contiguous slice of the chunk's entries and caches 64 plan entries per vector
+// load. A wave step covers 8 entries (8 TOD lines of 32 samples): lane 8*li + q handles samples
+// 4q..4q+3 of line li, loaded as float4.
 //
-// map2tod keeps a zero-padded 66x66 tile per plane so that corners outside the cell read 0
-// without a branch; lines touching several cells are zeroed first and then accumulated with
-// float atomics (reference: src_lib/map2tod.cu:15-62, 81-191).
+// Ownership. A sample belongs to the cell of its (iy0, ix0) corner; that cell is in the plan
+// for the sample's line, since the plan lists every corner's cell. The block of a cell works on
+// a 65x65 "halo" tile per plane: the cell plus the first row of the cell below and the first
+// column of the cell to the right (for periodic x, the column that wraparound reads), so it
+// evaluates or deposits all four corners of each sample it owns at tile offsets 0, 1, 65, 66,
+// and skips the samples it does not own. Lines that touch several cells are thus written once
+// per sample by plain stores: no zeroing pass and no atomics on the TOD (the reference zeroes
+// such lines and accumulates them atomically, src_lib/map2tod.cu:15-62, 81-191).
 //
 // tod2map accumulates in LDS with 64-bit integer atomics on fixed-point values: float LDS
-// atomics (ds_add_f32) run an order of magnitude slower than integer ones on this GPU. A pre-pass
-// finds max|tod|; with e = its binary exponent, contributions are scaled by S = 2^(27-e). A lane
-// sums its consecutive samples that share the same (iy0, ix0) in float registers (at most 4, so
-// |sum| < 2^30), converts the run's 12 partials to int32 and adds them into an int64 tile; the
-// flush converts each pixel once, float(sum)/S. Pixel sums are exact in fixed point, so the
-// result is more accurate than float32 accumulation. Registers take the runs of equal pixels
-// that would otherwise make lanes of one atomic instruction collide on the same LDS address.
-// If the TOD holds Inf/NaN the kernel falls back to float LDS atomics.
+// atomics (ds_add_f32) run an order of magnitude slower than integer ones on this GPU. Deposits
+// are w*t*S with S = 2^(27-e) for a bound 2^(e+1) on |t|, so each converts exactly to int32 and
+// pixel sums are exact in the int64 tile; the flush converts each pixel once, float(sum)/S,
+// which is more accurate than float32 accumulation. The bound is estimated in the kernel (4x
+// the largest |t| in the first lines of the chunk); a chunk holding a larger value runs again
+// with its exact maximum, and one holding Inf/NaN runs again with float LDS atomics. The halo
+// row and column, and the cell's own first row and column (other blocks' halo targets), are
+// flushed to the map with atomics, the rest with a plain read-modify-write.
 // Reference: src_lib/tod2map.cu:33-154.
 //
 // Both kernels also take per-detector gains, float response[2][ndet] (row 0 scales T, row 1
@@ -33,15 +36,12 @@ namespace gmm {
 namespace {
 
 constexpr int WAVE = 64;
-constexpr int TP = 66;              // padded tile side (map2tod)
-constexpr int TPLANE = TP * TP;     // floats per padded plane
+constexpr int HS = 65;              // halo tile side: the cell plus one row and column of neighbours
+constexpr int HPLANE = HS * HS;     // floats per halo tile plane
 
 constexpr int M2T_THREADS = 512;
-constexpr int M2T_U = 2;            // 8-line groups in flight per wave
 constexpr int T2M_THREADS = 1024;
-constexpr int T2M_U = 2;            // 8-line groups in flight per wave
 
-constexpr int ABSMAX_BLOCKS = 1024;
 constexpr uint32_t FINITE_LIMIT = 0x7f800000u;   // bits of +Inf
 
 __device__ inline int clamp_i(int v, int lo, int hi) { return min(max(v, lo), hi); }
@@ -103,7 +103,6 @@ __device__ inline float4 load4(const float *__restrict__ p)
     return make_float4(p[0], p[1], p[2], p[3]);
 }
 
-__device__ inline float comp(const float4 &v, int k) { return (k == 0) ? v.x : (k == 1) ? v.y : (k == 2) ? v.z : v.w; }
 
 // TOD element access, four consecutive samples at a time. TT is float, _Float16 or __bf16.
 // 16-bit TODs are always accessed with 8-byte vector operations (the host requires an 8-byte
@@ -227,38 +226,184 @@ __device__ inline void quad_scale(const Gains &g, uint32_t s0, float (&sc)[4])
     }
 }
 
-// Bilinear T + cos2a Q + sin2a U of one sample from the padded tile (corners outside the cell
-// read the zero border).
-template<bool Periodic, bool HasResp>
-__device__ inline float m2t_sample(const float *__restrict__ tile, float y, float x, float alpha,
-                                   int ny, int nx, int ybase, int xbase, float tr, float pr, uint32_t &err)
+// ------------------------------------------------------------------------------------------
+// Sample location for the operators: the same index and offset arithmetic as locate_pixel()
+// (so the kernels agree with the planner bit for bit), with the range checks deferred. Per lane,
+//   chk_y = max(-y, y - (ny-1)) and chk_x = max(lo - x, x - hi) over its samples (> 0 iff some
+//   sample is out of range; the signs of these exact-sign subtractions are exact), and
+//   nan_y, nan_x = sums of the coordinates (NaN iff some coordinate is NaN, or infinite ones
+//   of both signs, which the range check flags anyway).
+
+__device__ inline int cvt_rz(float v)
 {
-    float sn, cs;
-    sincos_angle(2.0f * alpha, sn, cs);
+    // v_cvt_i32_f32 rounds toward zero, saturates and maps NaN to 0 (as trunc_to_int).
+    int r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+
+struct Checks
+{
+    float y = -INFINITY, x = -INFINITY, ny = 0.0f, nx = 0.0f;
+
+    __device__ uint32_t bits() const
+    {
+        uint32_t e = 0;
+        if (!(y <= 0.0f) || !(ny == ny))
+            e |= ERR_BAD_YPIX;
+        if (!(x <= 0.0f) || !(nx == nx))
+            e |= ERR_BAD_XPIX;
+        return e;
+    }
+};
+
+struct Geom
+{
+    int ny, nx;
+    float ymax;     // ny - 1
+    float xlo, xhi; // valid x range: [0, nx-2] or, periodic, [-nx, 2nx]
+    float fnx;
+};
+
+__device__ inline Geom make_geom(int ny, int nx, bool periodic)
+{
+    Geom g;
+    g.ny = ny;
+    g.nx = nx;
+    g.ymax = float(ny - 1);
+    g.fnx = float(nx);
+    g.xlo = periodic ? -float(nx) : 0.0f;
+    g.xhi = periodic ? float(2 * nx) : float(nx - 2);
+    return g;
+}
+
+template<bool Periodic>
+__device__ inline void locate_op(float y, float x, const Geom &g, Checks &c, int &iy0, int &ix0, float &dy, float &dx)
+{
+    c.y = fmaxf(c.y, fmaxf(-y, y - g.ymax));
+    c.x = fmaxf(c.x, fmaxf(g.xlo - x, x - g.xhi));
+    c.ny += y;
+    c.nx += x;
+
+    int iy = cvt_rz(y);
+    iy = min(max(iy, 0), g.ny - 2);
+    dy = y - float(iy);
+    iy0 = iy;
+
+    if (Periodic) {
+        x = (x >= 0.0f) ? x : (x + g.fnx);
+        x = (x <= g.fnx) ? x : (x - g.fnx);
+        int ix = max(cvt_rz(x), 0);
+        dx = x - float(ix);
+        ix0 = (ix < g.nx) ? ix : 0;
+    }
+    else {
+        int ix = min(max(cvt_rz(x), 0), g.nx - 2);
+        dx = x - float(ix);
+        ix0 = ix;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Halo tile: the block's cell (64x64 per plane) plus row 64 and column W from the neighbouring
+// cells below and to the right, where W = 64, or, in periodic mode for the last cell column when
+// nx is not a multiple of 64, W = (nx-1) % 64 + 1: the column that x-wraparound (ix1 = 0) reads.
+// Absent cells read as zeros. With this tile, the block that owns a sample (the cell of its
+// (iy0, ix0) corner) evaluates all four corners at tile offsets 0, 1, HS, HS+1.
+
+struct HaloCells
+{
+    int64_t own, down, right, diag;   // cell offsets (-1 = absent)
+    int w;                            // halo column
+};
+
+__device__ inline int64_t cell_off(const int64_t *__restrict__ cell_offsets, int cy, int cx, int nycells, int nxcells)
+{
+    return (cy < nycells && cx < nxcells) ? cell_offsets[int64_t(cy) * nxcells + cx] : int64_t(-1);
+}
+
+__device__ inline HaloCells halo_cells(uint32_t icell, const int64_t *__restrict__ cell_offsets, int nycells,
+                                       int nxcells, int nx, bool periodic)
+{
+    const int cy = int(icell >> 10), cx = int(icell & 1023u);
+    const int last_cx = (nx - 1) >> 6;
+    const bool wrap = periodic && (cx == last_cx);
+    const int rcx = wrap ? 0 : cx + 1;
+    HaloCells h;
+    h.own = cell_off(cell_offsets, cy, cx, nycells, nxcells);
+    h.down = cell_off(cell_offsets, cy + 1, cx, nycells, nxcells);
+    h.right = cell_off(cell_offsets, cy, rcx, nycells, nxcells);
+    h.diag = cell_off(cell_offsets, cy + 1, rcx, nycells, nxcells);
+    h.w = wrap ? ((nx - 1) & 63) + 1 : 64;
+    return h;
+}
+
+template<int NT>
+__device__ inline void stage_halo_tile(float *__restrict__ tile, const float *__restrict__ lmap, const HaloCells &h,
+                                       int64_t ystride, int64_t polstride)
+{
+    // Interior (column W excluded when W < 64: it holds the wrapped column).
+    for (int i = threadIdx.x; i < 3 * CELL_NPIX; i += NT) {
+        const int p = i >> 12, r = (i >> 6) & 63, c = i & 63;
+        if (c == h.w)
+            continue;
+        tile[p * HPLANE + r * HS + c] = (h.own >= 0) ? lmap[h.own + p * polstride + r * ystride + c] : 0.0f;
+    }
+    // Row 64 (columns 0..W) and column W (rows 0..63).
+    const int nrow = h.w + 1, nper = nrow + 64;
+    for (int i = threadIdx.x; i < 3 * nper; i += NT) {
+        const int p = i / nper, j = i - p * nper;
+        int64_t src;
+        int idx;
+        if (j < nrow) {
+            idx = 64 * HS + j;
+            src = (j < h.w) ? ((h.down >= 0) ? h.down + j : -1) : h.diag;
+        }
+        else {
+            const int r = j - nrow;
+            idx = r * HS + h.w;
+            src = (h.right >= 0) ? h.right + r * ystride : -1;
+        }
+        tile[p * HPLANE + idx] = (src >= 0) ? lmap[src + p * polstride] : 0.0f;
+    }
+}
+
+// T + cos2a Q + sin2a U (times the gains), bilinear over the four corners at tile offset 'o'.
+template<bool HasResp>
+__device__ inline float m2t_interp(const float *__restrict__ tile, int o, float dy, float dx, float cs, float sn,
+                                   float tr, float pr)
+{
+    const float *tq = tile + HPLANE, *tu = tile + 2 * HPLANE;
     if (HasResp) {
         cs *= pr;
         sn *= pr;
     }
-    Pixel<float> p;
-    err |= locate_pixel(y, x, ny, nx, Periodic, p);
+    float t00 = tile[o], t01 = tile[o + 1], t10 = tile[o + HS], t11 = tile[o + HS + 1];
+    if (HasResp) {
+        t00 *= tr; t01 *= tr; t10 *= tr; t11 *= tr;
+    }
+    const float v00 = fmaf(sn, tu[o], fmaf(cs, tq[o], t00));
+    const float v01 = fmaf(sn, tu[o + 1], fmaf(cs, tq[o + 1], t01));
+    const float v10 = fmaf(sn, tu[o + HS], fmaf(cs, tq[o + HS], t10));
+    const float v11 = fmaf(sn, tu[o + HS + 1], fmaf(cs, tq[o + HS + 1], t11));
+    const float top = fmaf(dx, v01 - v00, v00);
+    const float bot = fmaf(dx, v11 - v10, v10);
+    return fmaf(dy, bot - top, top);
+}
 
-    const int r0 = clamp_i(p.iy0 - ybase + 1, 0, TP - 1) * TP;
-    const int r1 = clamp_i(p.iy1 - ybase + 1, 0, TP - 1) * TP;
-    const int c0 = clamp_i(p.ix0 - xbase + 1, 0, TP - 1);
-    const int c1 = clamp_i(p.ix1 - xbase + 1, 0, TP - 1);
-    const float *tq = tile + TPLANE, *tu = tile + 2 * TPLANE;
-
-    const float t00 = HasResp ? tr * tile[r0 + c0] : tile[r0 + c0];
-    const float t01 = HasResp ? tr * tile[r0 + c1] : tile[r0 + c1];
-    const float t10 = HasResp ? tr * tile[r1 + c0] : tile[r1 + c0];
-    const float t11 = HasResp ? tr * tile[r1 + c1] : tile[r1 + c1];
-    const float v00 = t00 + cs * tq[r0 + c0] + sn * tu[r0 + c0];
-    const float v01 = t01 + cs * tq[r0 + c1] + sn * tu[r0 + c1];
-    const float v10 = t10 + cs * tq[r1 + c0] + sn * tu[r1 + c0];
-    const float v11 = t11 + cs * tq[r1 + c1] + sn * tu[r1 + c1];
-
-    const float wy0 = 1.0f - p.dy, wx0 = 1.0f - p.dx;
-    return (wy0 * wx0) * v00 + (wy0 * p.dx) * v01 + (p.dy * wx0) * v10 + (p.dy * p.dx) * v11;
+template<bool Periodic, bool HasResp>
+__device__ inline float m2t_sample(const float *__restrict__ tile, float y, float x, float alpha, const Geom &g,
+                                   int ybase, int xbase, float tr, float pr, Checks &chk, bool &owned)
+{
+    int iy0, ix0;
+    float dy, dx;
+    locate_op<Periodic>(y, x, g, chk, iy0, ix0, dy, dx);
+    const int ly = iy0 - ybase, lx = ix0 - xbase;
+    owned = (unsigned(ly) < 64u) && (unsigned(lx) < 64u);
+    float sn, cs;
+    sincos_angle(2.0f * alpha, sn, cs);
+    const int o = owned ? (ly * HS + lx) : 0;
+    return m2t_interp<HasResp>(tile, o, dy, dx, cs, sn, tr, pr);
 }
 
 template<bool Periodic, bool Vec, bool HasResp, typename TT>
@@ -269,180 +414,133 @@ planned_map2tod_kernel(TT *__restrict__ tod, const float *__restrict__ lmap, con
                        int64_t nsamp, int ny, int nx, int nycells, int nxcells,
                        int64_t ystride, int64_t polstride, bool partial, bool accum, Gains gn)
 {
-    __shared__ float tile[3 * TPLANE];
+    __shared__ float tile[3 * HPLANE];
 
     const Chunk ch = chunks[blockIdx.x];
-    const int64_t off = chunk_cell_offset(ch, cell_offsets, nycells, nxcells);
+    const HaloCells h = halo_cells(ch.icell, cell_offsets, nycells, nxcells, nx, Periodic);
     const uint32_t ncl = uint32_t(nsamp >> 5);
-    uint32_t err = (off < 0 && !partial) ? ERR_NOT_IN_PIXELIZATION : 0u;
+    uint32_t err = (h.own < 0 && !partial) ? ERR_NOT_IN_PIXELIZATION : 0u;
 
-    if (off >= 0) {
-        // Interior: 3 planes x 64 rows x 64 columns, one row of 64 floats per wave load.
-        for (int i = threadIdx.x; i < 3 * CELL_NPIX; i += M2T_THREADS) {
-            int p = i >> 12, r = (i >> 6) & 63, c = i & 63;
-            tile[p * TPLANE + (r + 1) * TP + (c + 1)] = lmap[off + p * polstride + r * ystride + c];
-        }
-        // Zero border: rows 0 and 65 (66 entries each), columns 0 and 65 of rows 1..64.
-        for (int i = threadIdx.x; i < 3 * 260; i += M2T_THREADS) {
-            int p = i / 260, j = i % 260, idx;
-            if (j < 66)
-                idx = j;
-            else if (j < 132)
-                idx = 65 * TP + (j - 66);
-            else if (j < 196)
-                idx = (j - 132 + 1) * TP;
-            else
-                idx = (j - 196 + 1) * TP + 65;
-            tile[p * TPLANE + idx] = 0.0f;
-        }
-    }
+    stage_halo_tile<M2T_THREADS>(tile, lmap, h, ystride, polstride);
     __syncthreads();
 
+    const Geom g = make_geom(ny, nx, Periodic);
+    Checks chk;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane >> 3, q = lane & 7;
     const int ybase = int(ch.icell >> 10) << 6, xbase = int(ch.icell & 1023u) << 6;
+    using IO = TodIO<TT, Vec>;
 
     uint32_t wb, we;
     wave_slice(ch, wave, M2T_THREADS / WAVE, wb, we);
 
+    // Every sample is written once, by the block that owns it: no atomics and no zeroing.
     for (uint32_t base = wb; base < we; base += 64) {
         const uint32_t cnt = min(64u, we - base);
         const uint32_t packed = (uint32_t(lane) < cnt) ? pack_entry(plan_mt[base + lane]) : 0u;
 
-        // Variants with gains keep one line group in flight (their extra loads would spill).
-        constexpr int UU = (HasResp || !std::is_same<TT, float>::value) ? 1 : M2T_U;
-        using IO = TodIO<TT, Vec>;
-        for (uint32_t j = 0; j < cnt; j += 8 * UU) {
-            float4 y4[UU], x4[UU], a4[UU];
-            int64_t s0[UU];
-            bool act[UU], mfl[UU];
-#pragma unroll
-            for (int u = 0; u < UU; u++) {
-                const uint32_t idx = j + 8 * u + li;
-                const uint32_t e = __shfl(packed, int(min(idx, 63u)), WAVE);
-                const uint32_t icl = e & ICL_MASK;
-                act[u] = (idx < cnt) && (icl < ncl);
-                if (idx < cnt && icl >= ncl)
-                    err |= ERR_INCONSISTENT_NMT;
-                mfl[u] = (e >> 31) != 0;
-                s0[u] = int64_t(act[u] ? icl : 0u) * LINE + 4 * q;
-                y4[u] = load4<Vec>(xp + s0[u]);
-                x4[u] = load4<Vec>(xp + nsamp + s0[u]);
-                a4[u] = load4<Vec>(xp + 2 * nsamp + s0[u]);
+        for (uint32_t j = 0; j < cnt; j += 8) {
+            const uint32_t idx = j + li;
+            const uint32_t e = __shfl(packed, int(min(idx, 63u)), WAVE);
+            const uint32_t icl = e & ICL_MASK;
+            const bool act = (idx < cnt) && (icl < ncl);
+            if (idx < cnt && icl >= ncl)
+                err |= ERR_INCONSISTENT_NMT;
+            if (!act)
+                continue;
+            const int64_t s0 = int64_t(icl) * LINE + 4 * q;
+            const float4 y4 = load4<Vec>(xp + s0);
+            const float4 x4 = load4<Vec>(xp + nsamp + s0);
+            const float4 a4 = load4<Vec>(xp + 2 * nsamp + s0);
+            float tr[4], pr[4], sc[4];
+            quad_resp<HasResp>(gn, uint32_t(s0), tr, pr);
+            quad_scale<TT>(gn, uint32_t(s0), sc);
+
+            bool o0, o1, o2, o3;
+            float4 v;
+            // Scheduling barriers keep the four samples from being interleaved (which would
+            // exceed the register budget of 6 waves per SIMD).
+            v.x = m2t_sample<Periodic, HasResp>(tile, y4.x, x4.x, a4.x, g, ybase, xbase, tr[0], pr[0], chk, o0);
+            __builtin_amdgcn_sched_barrier(0);
+            v.y = m2t_sample<Periodic, HasResp>(tile, y4.y, x4.y, a4.y, g, ybase, xbase, tr[1], pr[1], chk, o1);
+            __builtin_amdgcn_sched_barrier(0);
+            v.z = m2t_sample<Periodic, HasResp>(tile, y4.z, x4.z, a4.z, g, ybase, xbase, tr[2], pr[2], chk, o2);
+            __builtin_amdgcn_sched_barrier(0);
+            v.w = m2t_sample<Periodic, HasResp>(tile, y4.w, x4.w, a4.w, g, ybase, xbase, tr[3], pr[3], chk, o3);
+            if constexpr (!std::is_same<TT, float>::value) {
+                v.x /= sc[0]; v.y /= sc[1]; v.z /= sc[2]; v.w /= sc[3];
             }
-#pragma unroll
-            for (int u = 0; u < UU; u++) {
-                if (!act[u])
-                    continue;
-                if (off < 0) {
-                    if (!accum && !mfl[u])
-                        IO::store(tod + s0[u], make_float4(0.f, 0.f, 0.f, 0.f));
-                    continue;
-                }
-                float tr[4], pr[4], sc[4], vv[4];
-                quad_resp<HasResp>(gn, uint32_t(s0[u]), tr, pr);
-                quad_scale<TT>(gn, uint32_t(s0[u]), sc);
-#pragma unroll 2
-                for (int k = 0; k < 4; k++) {
-                    vv[k] = m2t_sample<Periodic, HasResp>(tile, comp(y4[u], k), comp(x4[u], k), comp(a4[u], k),
-                                                          ny, nx, ybase, xbase, tr[k], pr[k], err);
-                    if constexpr (!std::is_same<TT, float>::value)
-                        vv[k] = vv[k] / sc[k];
-                }
-                const float4 v = make_float4(vv[0], vv[1], vv[2], vv[3]);
-                TT *t = tod + s0[u];
-                if (mfl[u])
-                    IO::atomic_add(t, v);
-                else if (accum) {
+            TT *t = tod + s0;
+            if (o0 && o1 && o2 && o3) {
+                if (accum) {
                     const float4 o = IO::load(t);
-                    IO::store(t, make_float4(o.x + v.x, o.y + v.y, o.z + v.z, o.w + v.w));
+                    v.x += o.x; v.y += o.y; v.z += o.z; v.w += o.w;
                 }
-                else
-                    IO::store(t, v);
+                IO::store(t, v);
+            }
+            else if (o0 || o1 || o2 || o3) {
+                // A quad shared with another cell: store this block's samples one by one.
+                if (o0) t[0] = TT(accum ? float(t[0]) + v.x : v.x);
+                if (o1) t[1] = TT(accum ? float(t[1]) + v.y : v.y);
+                if (o2) t[2] = TT(accum ? float(t[2]) + v.z : v.z);
+                if (o3) t[3] = TT(accum ? float(t[3]) + v.w : v.w);
             }
         }
     }
 
-    report_err(err, err_word);
-}
-
-// Zeroes the TOD lines that touch more than one cell (they are accumulated atomically).
-template<typename TT>
-__global__ void zero_lines_kernel(TT *__restrict__ tod, const uint32_t *__restrict__ mlines, int64_t nml)
-{
-    const int64_t n = nml * LINE;
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
-        tod[int64_t(mlines[i >> 5]) * LINE + (i & 31)] = TT(0.0f);
+    report_err(err | chk.bits(), err_word);
 }
 
 // ------------------------------------------------------------------------------------------
 
-// Per-block maxima, as float bits (NaN > Inf > finite in this order), over ABSMAX_BLOCKS
-// blocks: partial[b] = max|tod| over block b's share of the samples, partial[B + b] = max|response|
-// and partial[2B + b] = max|scale| over its share of those arrays (0 where absent).
-template<typename TT, bool Vec>
-__global__ void __launch_bounds__(256)
-tod_absmax_kernel(const TT *__restrict__ tod, int64_t n, uint32_t *__restrict__ partial,
-                  const float *__restrict__ resp, int64_t nresp, const float *__restrict__ scale, int64_t nscale)
+// Fixed-point deposit of w*v (already scaled) into a 64-bit LDS accumulator.
+__device__ inline void dep_fixed(unsigned long long *__restrict__ a, float v)
 {
-    __shared__ uint32_t s_w[3][4];
-    const int64_t stride = int64_t(gridDim.x) * 256, i0 = int64_t(blockIdx.x) * 256 + threadIdx.x;
-    uint32_t m = 0, mr = 0, ms = 0;
-    if (Vec || !std::is_same<TT, float>::value) {
-        for (int64_t i = i0; i < (n >> 2); i += stride) {
-            const float4 v = TodIO<TT, true>::load(tod + 4 * i);
-            m = max(m, __float_as_uint(v.x) & 0x7fffffffu);
-            m = max(m, __float_as_uint(v.y) & 0x7fffffffu);
-            m = max(m, __float_as_uint(v.z) & 0x7fffffffu);
-            m = max(m, __float_as_uint(v.w) & 0x7fffffffu);
+    const int iv = cvt_rz(v);
+    atomicAdd(a, (unsigned long long)(long long)iv);
+}
+
+// One sample's contributions to the four corners at halo-tile offset o: T, Q, U planes.
+template<bool Fixed>
+__device__ inline void deposit4(unsigned long long *__restrict__ acc, float *__restrict__ facc, int o, float dy,
+                                float dx, float t, float tq, float tu)
+{
+    const float wy0 = 1.0f - dy, wx0 = 1.0f - dx;
+    const float w[4] = { wy0 * wx0, wy0 * dx, dy * wx0, dy * dx };
+    const int ofs[4] = { 0, 1, HS, HS + 1 };
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int k = o + ofs[c];
+        if (Fixed) {
+            dep_fixed(acc + k, w[c] * t);
+            dep_fixed(acc + HPLANE + k, w[c] * tq);
+            dep_fixed(acc + 2 * HPLANE + k, w[c] * tu);
+        }
+        else {
+            atomicAdd(facc + k, w[c] * t);
+            atomicAdd(facc + HPLANE + k, w[c] * tq);
+            atomicAdd(facc + 2 * HPLANE + k, w[c] * tu);
         }
     }
-    else {
-        for (int64_t i = i0; i < n; i += stride)
-            m = max(m, __float_as_uint(float(tod[i])) & 0x7fffffffu);
-    }
-    for (int64_t i = i0; i < nresp; i += stride)
-        mr = max(mr, __float_as_uint(resp[i]) & 0x7fffffffu);
-    for (int64_t i = i0; i < nscale; i += stride)
-        ms = max(ms, __float_as_uint(scale[i]) & 0x7fffffffu);
-    m = wave_max(m);
-    mr = wave_max(mr);
-    ms = wave_max(ms);
-    if ((threadIdx.x & 63) == 0) {
-        s_w[0][threadIdx.x >> 6] = m;
-        s_w[1][threadIdx.x >> 6] = mr;
-        s_w[2][threadIdx.x >> 6] = ms;
-    }
-    __syncthreads();
-    if (threadIdx.x < 3) {
-        const uint32_t *w = s_w[threadIdx.x];
-        partial[threadIdx.x * gridDim.x + blockIdx.x] = max(max(w[0], w[1]), max(w[2], w[3]));
-    }
 }
 
-__device__ inline void acc_add3(unsigned long long *__restrict__ acc, int k, float a, float b, float c)
+// One sample of tod2map: locate; if the block owns it (its (iy0, ix0) corner lies in the
+// block's cell), deposit all four corners into the halo tile.
+template<bool Periodic, bool HasResp, bool Fixed>
+__device__ inline void t2m_sample(unsigned long long *__restrict__ acc, float *__restrict__ facc, float y, float x,
+                                  float alpha, float t, const Geom &g, int ybase, int xbase, float tr, float pr,
+                                  Checks &chk)
 {
-    atomicAdd(acc + k, (unsigned long long)(long long)int(a));
-    atomicAdd(acc + CELL_NPIX + k, (unsigned long long)(long long)int(b));
-    atomicAdd(acc + 2 * CELL_NPIX + k, (unsigned long long)(long long)int(c));
-}
-
-// Adds one run's register partials (12 values: corners 00, 01, 10, 11 x T, Q, U; already
-// scaled) for the corners that lie inside the cell.
-__device__ inline void flush_run(unsigned long long *__restrict__ acc, int ry0, int ry1, int rx0, int rx1,
-                                 const float (&a)[12])
-{
-    if (unsigned(ry0) < 64u) {
-        if (unsigned(rx0) < 64u)
-            acc_add3(acc, (ry0 << 6) | rx0, a[0], a[1], a[2]);
-        if (unsigned(rx1) < 64u)
-            acc_add3(acc, (ry0 << 6) | rx1, a[3], a[4], a[5]);
-    }
-    if (unsigned(ry1) < 64u) {
-        if (unsigned(rx0) < 64u)
-            acc_add3(acc, (ry1 << 6) | rx0, a[6], a[7], a[8]);
-        if (unsigned(rx1) < 64u)
-            acc_add3(acc, (ry1 << 6) | rx1, a[9], a[10], a[11]);
+    int iy0, ix0;
+    float dy, dx;
+    locate_op<Periodic>(y, x, g, chk, iy0, ix0, dy, dx);
+    const int ly = iy0 - ybase, lx = ix0 - xbase;
+    if ((unsigned(ly) < 64u) && (unsigned(lx) < 64u)) {
+        float sn, cs;
+        sincos_angle(2.0f * alpha, sn, cs);
+        const float ts = HasResp ? t * tr : t;
+        const float ps = HasResp ? t * pr : t;
+        deposit4<Fixed>(acc, facc, ly * HS + lx, dy, dx, ts, ps * cs, ps * sn);
     }
 }
 
@@ -450,216 +548,209 @@ template<bool Periodic, bool Vec, bool HasResp, typename TT>
 __global__ void __launch_bounds__(T2M_THREADS)
 planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, const float *__restrict__ xp,
                        const int64_t *__restrict__ cell_offsets, const uint64_t *__restrict__ plan_mt,
-                       const Chunk *__restrict__ chunks, const uint32_t *__restrict__ absmax_partial,
-                       uint32_t *__restrict__ err_word, int64_t nsamp, int ny, int nx, int nycells, int nxcells,
-                       int64_t ystride, int64_t polstride, bool partial, Gains gn)
+                       const Chunk *__restrict__ chunks, uint32_t *__restrict__ err_word, int64_t nsamp, int ny,
+                       int nx, int nycells, int nxcells, int64_t ystride, int64_t polstride, bool partial, Gains gn)
 {
     constexpr bool Q16 = !std::is_same<TT, float>::value;
     using IO = TodIO<TT, Vec>;
-    __shared__ unsigned long long acc[3 * CELL_NPIX];    // 96 KiB
-    __shared__ uint32_t s_red[3][T2M_THREADS / WAVE];
-    __shared__ uint32_t s_nonfinite;
+    __shared__ unsigned long long acc[3 * HPLANE];       // 99 KiB
+    __shared__ uint32_t s_max, s_redo;
     float *facc = reinterpret_cast<float *>(acc);         // float tile for the non-finite fallback
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const Chunk ch = chunks[blockIdx.x];
-    const int64_t off = chunk_cell_offset(ch, cell_offsets, nycells, nxcells);
+    const HaloCells h = halo_cells(ch.icell, cell_offsets, nycells, nxcells, nx, Periodic);
     const uint32_t ncl = uint32_t(nsamp >> 5);
-    uint32_t err = (off < 0 && !partial) ? ERR_NOT_IN_PIXELIZATION : 0u;
+    uint32_t err = (h.own < 0 && !partial) ? ERR_NOT_IN_PIXELIZATION : 0u;
 
-    if (off < 0) {
-        // Samples whose cells are not in the local map are dropped (tod2map_reference.cu:14-46
-        // skips them too); only the error bit is reported.
-        report_err(err, err_word);
-        return;
-    }
+    // Samples owned by an absent cell still deposit into present neighbours through the halo,
+    // so every block runs; cells that are absent receive nothing (tod2map_reference.cu:14-46
+    // skips them too).
 
-    // Scale from the bound B = (max|t| * max|scale|) * max|response|, evaluated in the order the
-    // samples are (t * scale, then * response), so that by monotonic rounding every deposited
-    // |t| <= B. ABSMAX_BLOCKS == T2M_THREADS partials of each kind, one per thread. A lane sums
-    // up to 4 samples in registers before converting, so |w*t*S| < 2^30 keeps the int32 safe.
-    // A 16-bit TOD needs no pass over the samples: every finite value is at most the type's
-    // largest finite value, which stands in for max|t|; non-finite values are detected while
-    // accumulating and send the block through a second, floating-point pass.
-    if (threadIdx.x == 0)
-        s_nonfinite = 0;
-    {
-        const uint32_t mt = wave_max(absmax_partial[threadIdx.x]);
-        const uint32_t mr = HasResp ? wave_max(absmax_partial[ABSMAX_BLOCKS + threadIdx.x]) : 0u;
-        const uint32_t ms = Q16 ? wave_max(absmax_partial[2 * ABSMAX_BLOCKS + threadIdx.x]) : 0u;
-        if (lane == 0) {
-            s_red[0][wave] = mt;
-            s_red[1][wave] = mr;
-            s_red[2][wave] = ms;
-        }
-    }
-    for (int i = threadIdx.x; i < 3 * CELL_NPIX; i += T2M_THREADS)
-        acc[i] = 0ull;
-    __syncthreads();
-    uint32_t m = 0, mr = 0, ms = 0;
-#pragma unroll
-    for (int w = 0; w < T2M_THREADS / WAVE; w++) {
-        m = max(m, s_red[0][w]);
-        mr = max(mr, s_red[1][w]);
-        ms = max(ms, s_red[2][w]);
-    }
-    if (Q16)
-        m = std::is_same<TT, _Float16>::value ? 0x477fe000u : 0x7f7f0000u;   // 65504, 0x1.fep127
-    bool finite = (m < FINITE_LIMIT);
-    if (Q16) {
-        finite = finite && (ms < FINITE_LIMIT);
-        m = __float_as_uint(__uint_as_float(m) * __uint_as_float(ms));
-    }
-    if (HasResp) {
-        finite = finite && (mr < FINITE_LIMIT);
-        m = __float_as_uint(__uint_as_float(m) * __uint_as_float(mr));
-    }
-    finite = finite && (m < FINITE_LIMIT);
-    const int e = int(m >> 23) - 127;                    // binary exponent of max|t|
-    const int sexp = clamp_i(27 - e, -100, 126);
-    const float scale = __uint_as_float(uint32_t(sexp + 127) << 23);
-    const float inv_scale = __uint_as_float(uint32_t(127 - sexp) << 23);
-
+    const Geom g = make_geom(ny, nx, Periodic);
     const int li = lane >> 3, q = lane & 7;
     const int ybase = int(ch.icell >> 10) << 6, xbase = int(ch.icell & 1023u) << 6;
-
     uint32_t wb, we;
     wave_slice(ch, wave, T2M_THREADS / WAVE, wb, we);
 
+    // Fixed-point scale. Deposits are w * t * S with S = 2^(27 - e) and |t| < 2^(e+1), so each
+    // converts exactly to int32 (|.| < 2^28) and the tile sums in int64. The exponent e comes
+    // from an estimate: 4 x the largest |t| (times the gains) among the first 8 lines of every
+    // wave's slice. Each lane tracks the largest |t| it deposits; if any exceeds the estimate's
+    // range, or is not finite, the block discards its tile and runs its chunk again, with the
+    // exact maximum, or with float atomics when that maximum is not finite. No pass over the
+    // TOD is needed before the kernel.
+    if (threadIdx.x == 0) {
+        s_max = 0;
+        s_redo = 0;
+    }
+    __syncthreads();
+    {
+        float m = 0.0f;
+        const uint32_t idx = wb + uint32_t(li);
+        if (idx < we) {
+            const uint32_t icl = uint32_t(plan_mt[idx] >> ICL_SHIFT) & ICL_MASK;
+            if (icl < ncl) {
+                const int64_t s0 = int64_t(icl) * LINE + 4 * q;
+                const float4 t4 = IO::load(tod + s0);
+                float tr[4], pr[4], sc[4];
+                quad_resp<HasResp>(gn, uint32_t(s0), tr, pr);
+                quad_scale<TT>(gn, uint32_t(s0), sc);
+                const float tv[4] = { t4.x, t4.y, t4.z, t4.w };
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const float t = Q16 ? tv[k] * sc[k] : tv[k];
+                    m = fmaxf(m, fabsf(HasResp ? t * tr[k] : t));
+                    m = fmaxf(m, fabsf(HasResp ? t * pr[k] : t));
+                }
+            }
+        }
+        const uint32_t mw = wave_max(__float_as_uint(m));
+        if (lane == 0)
+            atomicMax(&s_max, mw);
+    }
+    for (int i = threadIdx.x; i < 3 * HPLANE; i += T2M_THREADS)
+        acc[i] = 0ull;
+    __syncthreads();
+    uint32_t mbits = s_max;
+    mbits = (mbits < FINITE_LIMIT) ? __float_as_uint(4.0f * __uint_as_float(mbits)) : mbits;
+    bool finite = (mbits < FINITE_LIMIT);
+
     for (;;) {
-        bool nonfinite = false;
+        const int e = int(mbits >> 23) - 127;            // binary exponent of the bound
+        const int sexp = clamp_i(27 - e, -100, 126);
+        const float scale = __uint_as_float(uint32_t(sexp + 127) << 23);
+        const float limit = __uint_as_float(uint32_t(e + 128) << 23);   // 2^(e+1)
+        float mrun = 0.0f;                               // largest |t| deposited by this lane
+        float tsum = 0.0f;                               // NaN iff some deposited t is NaN
+        Checks chk;
+
         for (uint32_t base = wb; base < we; base += 64) {
             const uint32_t cnt = min(64u, we - base);
             const uint32_t packed = (uint32_t(lane) < cnt) ? pack_entry(plan_mt[base + lane]) : 0u;
 
-            for (uint32_t j = 0; j < cnt; j += 8 * T2M_U) {
-                float4 y4[T2M_U], x4[T2M_U], a4[T2M_U], t4[T2M_U];
-                int64_t s0[T2M_U];
-                bool act[T2M_U];
-    #pragma unroll
-                for (int u = 0; u < T2M_U; u++) {
-                    const uint32_t idx = j + 8 * u + li;
-                    const uint32_t e2 = __shfl(packed, int(min(idx, 63u)), WAVE);
-                    const uint32_t icl = e2 & ICL_MASK;
-                    act[u] = (idx < cnt) && (icl < ncl);
-                    if (idx < cnt && icl >= ncl)
-                        err |= ERR_INCONSISTENT_NMT;
-                    s0[u] = int64_t(act[u] ? icl : 0u) * LINE + 4 * q;
-                    y4[u] = load4<Vec>(xp + s0[u]);
-                    x4[u] = load4<Vec>(xp + nsamp + s0[u]);
-                    a4[u] = load4<Vec>(xp + 2 * nsamp + s0[u]);
-                    t4[u] = IO::load(tod + s0[u]);
+            for (uint32_t j = 0; j < cnt; j += 8) {
+                const uint32_t idx = j + li;
+                const uint32_t e2 = __shfl(packed, int(min(idx, 63u)), WAVE);
+                const uint32_t icl = e2 & ICL_MASK;
+                const bool act = (idx < cnt) && (icl < ncl);
+                if (idx < cnt && icl >= ncl)
+                    err |= ERR_INCONSISTENT_NMT;
+                if (!act)
+                    continue;
+                const int64_t s0 = int64_t(icl) * LINE + 4 * q;
+                const float4 y4 = load4<Vec>(xp + s0);
+                const float4 x4 = load4<Vec>(xp + nsamp + s0);
+                const float4 a4 = load4<Vec>(xp + 2 * nsamp + s0);
+                float4 t4 = IO::load(tod + s0);
+                float tr[4], pr[4], sc[4];
+                quad_resp<HasResp>(gn, uint32_t(s0), tr, pr);
+                quad_scale<TT>(gn, uint32_t(s0), sc);
+                if (Q16) {
+                    t4.x *= sc[0];
+                    t4.y *= sc[1];
+                    t4.z *= sc[2];
+                    t4.w *= sc[3];
                 }
-    #pragma unroll
-                for (int u = 0; u < T2M_U; u++) {
-                    if (!act[u])
-                        continue;
-                    float tr[4], pr[4], sc[4];
-                    quad_resp<HasResp>(gn, uint32_t(s0[u]), tr, pr);
-                    quad_scale<TT>(gn, uint32_t(s0[u]), sc);
-                    if (Q16) {
-                        t4[u].x *= sc[0];
-                        t4[u].y *= sc[1];
-                        t4[u].z *= sc[2];
-                        t4[u].w *= sc[3];
-                        nonfinite |= !(isfinite(t4[u].x) && isfinite(t4[u].y) && isfinite(t4[u].z) &&
-                                       isfinite(t4[u].w));
+                if (finite) {
+                    // NaN makes fmaxf skip the value: count it through the sum instead.
+                    if (HasResp) {
+                        mrun = fmaxf(mrun, fmaxf(fabsf(t4.x * tr[0]), fabsf(t4.x * pr[0])));
+                        mrun = fmaxf(mrun, fmaxf(fabsf(t4.y * tr[1]), fabsf(t4.y * pr[1])));
+                        mrun = fmaxf(mrun, fmaxf(fabsf(t4.z * tr[2]), fabsf(t4.z * pr[2])));
+                        mrun = fmaxf(mrun, fmaxf(fabsf(t4.w * tr[3]), fabsf(t4.w * pr[3])));
                     }
-                    if (finite) {
-                        // Runs of samples with the same (iy0, ix0) are summed in registers; the LDS
-                        // atomics are issued once per run.
-                        float a[12];
-    #pragma unroll
-                        for (int i = 0; i < 12; i++)
-                            a[i] = 0.0f;
-                        uint32_t cur = 0xffffffffu;
-                        int ry0 = 0, ry1 = 0, rx0 = 0, rx1 = 0;
-    #pragma unroll
-                        for (int k = 0; k < 4; k++) {
-                            float sn, cs;
-                            sincos_angle(2.0f * comp(a4[u], k), sn, cs);
-                            Pixel<float> p;
-                            err |= locate_pixel(comp(y4[u], k), comp(x4[u], k), ny, nx, Periodic, p);
-                            const uint32_t key = (uint32_t(p.iy0) << 16) | uint32_t(p.ix0);
-                            if (key != cur) {
-                                if (k > 0) {
-                                    flush_run(acc, ry0, ry1, rx0, rx1, a);
-    #pragma unroll
-                                    for (int i = 0; i < 12; i++)
-                                        a[i] = 0.0f;
-                                }
-                                cur = key;
-                                ry0 = p.iy0 - ybase;
-                                ry1 = p.iy1 - ybase;
-                                rx0 = p.ix0 - xbase;
-                                rx1 = p.ix1 - xbase;
-                            }
-                            const float t0 = comp(t4[u], k) * scale;
-                            const float ts = HasResp ? t0 * tr[k] : t0;
-                            const float ps = HasResp ? t0 * pr[k] : t0;
-                            const float qs = ps * cs, us = ps * sn;
-                            const float wy0 = 1.0f - p.dy, wx0 = 1.0f - p.dx;
-                            const float w[4] = { wy0 * wx0, wy0 * p.dx, p.dy * wx0, p.dy * p.dx };
-    #pragma unroll
-                            for (int c = 0; c < 4; c++) {
-                                a[3 * c] += w[c] * ts;
-                                a[3 * c + 1] += w[c] * qs;
-                                a[3 * c + 2] += w[c] * us;
-                            }
-                        }
-                        flush_run(acc, ry0, ry1, rx0, rx1, a);
-                    }
-                    else {
-    #pragma unroll
-                        for (int k = 0; k < 4; k++) {
-                            float sn, cs;
-                            sincos_angle(2.0f * comp(a4[u], k), sn, cs);
-                            Pixel<float> p;
-                            err |= locate_pixel(comp(y4[u], k), comp(x4[u], k), ny, nx, Periodic, p);
-                            const float t = comp(t4[u], k) * tr[k], qv = comp(t4[u], k) * pr[k] * cs,
-                                        uv = comp(t4[u], k) * pr[k] * sn;
-                            const int ry[2] = { p.iy0 - ybase, p.iy1 - ybase };
-                            const int rx[2] = { p.ix0 - xbase, p.ix1 - xbase };
-                            const float wy[2] = { 1.0f - p.dy, p.dy };
-                            const float wx[2] = { 1.0f - p.dx, p.dx };
-                            for (int aa = 0; aa < 2; aa++)
-                                for (int bb = 0; bb < 2; bb++)
-                                    if ((unsigned(ry[aa]) < 64u) && (unsigned(rx[bb]) < 64u)) {
-                                        const float w = wy[aa] * wx[bb];
-                                        const int kk = (ry[aa] << 6) | rx[bb];
-                                        atomicAdd(facc + kk, w * t);
-                                        atomicAdd(facc + CELL_NPIX + kk, w * qv);
-                                        atomicAdd(facc + 2 * CELL_NPIX + kk, w * uv);
-                                    }
-                        }
-                    }
+                    else
+                        mrun = fmaxf(mrun, fmaxf(fmaxf(fabsf(t4.x), fabsf(t4.y)), fmaxf(fabsf(t4.z), fabsf(t4.w))));
+                    tsum += (t4.x + t4.y) + (t4.z + t4.w);
+                    t4.x *= scale;
+                    t4.y *= scale;
+                    t4.z *= scale;
+                    t4.w *= scale;
+                    t2m_sample<Periodic, HasResp, true>(acc, facc, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, tr[0], pr[0], chk);
+                    t2m_sample<Periodic, HasResp, true>(acc, facc, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, tr[1], pr[1], chk);
+                    t2m_sample<Periodic, HasResp, true>(acc, facc, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, tr[2], pr[2], chk);
+                    t2m_sample<Periodic, HasResp, true>(acc, facc, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, tr[3], pr[3], chk);
+                }
+                else {
+                    t2m_sample<Periodic, HasResp, false>(acc, facc, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, tr[0], pr[0], chk);
+                    t2m_sample<Periodic, HasResp, false>(acc, facc, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, tr[1], pr[1], chk);
+                    t2m_sample<Periodic, HasResp, false>(acc, facc, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, tr[2], pr[2], chk);
+                    t2m_sample<Periodic, HasResp, false>(acc, facc, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, tr[3], pr[3], chk);
                 }
             }
         }
-        if (!Q16 || !finite)
+        err |= chk.bits();
+        const bool lane_nan = !(tsum == tsum);
+        if (!finite)
             break;
-        // 16-bit TOD: restart the chunk in floating point if a non-finite value was seen.
-        if (nonfinite)
-            atomicOr(&s_nonfinite, 1u);
+        // Any value beyond the estimate's range or non-finite: redo the chunk.
+        const uint32_t mw = wave_max(lane_nan ? FINITE_LIMIT : __float_as_uint(mrun));
+        if (lane == 0 && mw >= __float_as_uint(limit))
+            atomicMax(&s_redo, mw);
         __syncthreads();
-        if (s_nonfinite == 0)
+        const uint32_t redo = s_redo;
+        if (redo == 0)
             break;
-        for (int i = threadIdx.x; i < 3 * CELL_NPIX; i += T2M_THREADS)
-            facc[i] = 0.0f;
-        finite = false;
+        // Redo with the exact maximum (now within range), or in floating point.
+        mbits = redo;
+        finite = (mbits < FINITE_LIMIT);
+        for (int i = threadIdx.x; i < 3 * HPLANE; i += T2M_THREADS) {
+            if (finite)
+                acc[i] = 0ull;
+            else if (i < (3 * HPLANE + 1) / 2)
+                acc[i] = 0ull;        // the float tile is the first half
+        }
         __syncthreads();
+        if (threadIdx.x == 0)
+            s_redo = 0;
     }
     __syncthreads();
+    const int sexp_f = clamp_i(27 - (int(mbits >> 23) - 127), -100, 126);
+    const float inv_scale = __uint_as_float(uint32_t(127 - sexp_f) << 23);
 
+    // Flush. The cell's row 0 and column 0 also receive other blocks' halo deposits (and column
+    // 0 of cell column 0 the wrapped ones), so they and every split cell are added atomically;
+    // the rest of the cell with a plain read-modify-write. Halo row 64 and column W go to the
+    // neighbouring cells, atomically; deposits into absent cells are dropped.
     const bool shared = (ch.flags & CHUNK_SHARED) != 0;
-    for (int i = threadIdx.x; i < 3 * CELL_NPIX; i += T2M_THREADS) {
-        const int pl = i >> 12, r = (i >> 6) & 63, c = i & 63;
-        float *g = lmap + off + pl * polstride + r * ystride + c;
-        const float v = finite ? float((long long)acc[i]) * inv_scale : facc[i];
-        if (shared)
-            unsafeAtomicAdd(g, v);
+    const int nrow = h.w + 1, nper = nrow + 64;
+    for (int i = threadIdx.x; i < 3 * (CELL_NPIX + nper); i += T2M_THREADS) {
+        int pl, k;
+        int64_t dst;
+        bool atomic = true;
+        if (i < 3 * CELL_NPIX) {
+            pl = i >> 12;
+            const int r = (i >> 6) & 63, c = i & 63;
+            if (c == h.w || h.own < 0)
+                continue;
+            k = r * HS + c;
+            dst = h.own + r * ystride + c;
+            atomic = shared || r == 0 || c == 0;
+        }
+        else {
+            const int i2 = i - 3 * CELL_NPIX;
+            pl = i2 / nper;
+            const int jj = i2 - pl * nper;
+            if (jj < nrow) {
+                k = 64 * HS + jj;
+                dst = (jj < h.w) ? ((h.down >= 0) ? h.down + jj : -1) : h.diag;
+            }
+            else {
+                const int r = jj - nrow;
+                k = r * HS + h.w;
+                dst = (h.right >= 0) ? h.right + r * ystride : -1;
+            }
+            if (dst < 0)
+                continue;
+        }
+        k += pl * HPLANE;
+        const float v = finite ? float((long long)acc[k]) * inv_scale : facc[k];
+        float *gp = lmap + dst + pl * polstride;
+        if (atomic)
+            unsafeAtomicAdd(gp, v);
         else
-            *g += v;
+            *gp += v;
     }
 
     report_err(err, err_word);
@@ -694,7 +785,7 @@ void launch_t2m(float *lmap, const TT *tod, const float *xp, const OpArgs &a, co
     auto k = lp.periodic_xcoord ? (vec ? planned_tod2map_kernel<true, true, HasResp, TT> : planned_tod2map_kernel<true, false, HasResp, TT>)
                                 : (vec ? planned_tod2map_kernel<false, true, HasResp, TT> : planned_tod2map_kernel<false, false, HasResp, TT>);
     kernel_timer_mark(TK_TOD2MAP, false, stream);
-    k<<<grid, block, 0, stream>>>(lmap, tod, xp, lp.offsets_gpu, plan.plan_mt, plan.chunks, plan.absmax, err_word,
+    k<<<grid, block, 0, stream>>>(lmap, tod, xp, lp.offsets_gpu, plan.plan_mt, plan.chunks, err_word,
                                   a.nsamp, int(lp.nypix_global), int(lp.nxpix_global), int(lp.nycells),
                                   int(lp.nxcells), lp.ystride, lp.polstride, a.partial, gn);
     GMM_HIP(hipGetLastError());
@@ -717,12 +808,6 @@ template<typename TT>
 void map2tod_typed(TT *tod, const float *lmap, const float *xp, const OpArgs &a, const gmm_plan &plan,
                    bool accum, const TodGains &g, hipStream_t stream)
 {
-    if (!accum && plan.nmlines > 0) {
-        int64_t n = plan.nmlines * LINE;
-        int grid = int(std::min<int64_t>((n + 255) / 256, 8192));
-        zero_lines_kernel<TT><<<grid, 256, 0, stream>>>(tod, plan.mlines, plan.nmlines);
-        GMM_HIP(hipGetLastError());
-    }
     if (plan.nchunks == 0)
         return;
     const Gains gn = make_gains(a, g);
@@ -738,20 +823,6 @@ void tod2map_typed(float *lmap, const TT *tod, const float *xp, const OpArgs &a,
 {
     if (plan.nchunks == 0)
         return;
-    static_assert(ABSMAX_BLOCKS == T2M_THREADS, "one max|t| partial per thread of the tod2map block");
-    const int64_t nresp = g.response ? 2 * g.resp_ndet : 0;
-    const int64_t nscale = g.scale ? g.scale_nrows : 0;
-    // A 16-bit TOD is bounded by its type (the kernel substitutes the largest finite value),
-    // so only the gain arrays are scanned.
-    const int64_t nt = std::is_same<TT, float>::value ? a.nsamp : 0;
-    if (std::is_same<TT, float>::value && (reinterpret_cast<uintptr_t>(tod) & 15) != 0)
-        tod_absmax_kernel<TT, false><<<ABSMAX_BLOCKS, 256, 0, stream>>>(tod, nt, plan.absmax, g.response, nresp,
-                                                                       g.scale, nscale);
-    else
-        tod_absmax_kernel<TT, true><<<ABSMAX_BLOCKS, 256, 0, stream>>>(tod, nt, plan.absmax, g.response, nresp,
-                                                                      g.scale, nscale);
-    GMM_HIP(hipGetLastError());
-
     const Gains gn = make_gains(a, g);
     if (g.response)
         launch_t2m<true, TT>(lmap, tod, xp, a, plan, gn, stream);
@@ -774,7 +845,7 @@ void launch_planned_map2tod(void *tod, const float *lmap, const float *xp, const
     }
 }
 
-int64_t tod2map_scratch_words() { return 3 * ABSMAX_BLOCKS; }
+int64_t tod2map_scratch_words() { return 0; }
 
 void launch_planned_tod2map(float *lmap, const void *tod, const float *xp, const OpArgs &a,
                             const gmm_plan &plan, const TodGains &g, hipStream_t stream)

@@ -613,6 +613,42 @@ PYBIND11_MODULE(_gpu_mm_amd_py, m)
         }
         return n;
     }, py::arg("global_ncells"), py::arg("local_pixelization"), py::arg("plan"), py::arg("stream") = 0);
+    m.def("cell_broadcast", [](py::object dst, py::object src, py::object index_map, uintptr_t stream) {
+        const char *where = "cell_broadcast";
+        Arr d = get_arr(dst, where, "dst"), sr = get_arr(src, where, "src"), im = get_arr(index_map, where, "index_map");
+        require(d, where, "dst", kDLFloat, 32, true, "float32");
+        require(sr, where, "src", kDLFloat, 32, true, "float32");
+        require(im, where, "index_map", kDLInt, 64, true, "int64");
+        if (im.shape.size() != 1 || d.size != im.size * 12288 || sr.size % 12288)
+            raise(std::string(where) + ": expected dst of 3*64*64*len(index_map) floats and src of whole cells");
+        py::gil_scoped_release nogil;
+        check(gmm_cell_broadcast(static_cast<float *>(d.ptr), im.size, static_cast<const float *>(sr.ptr),
+                                 sr.size / 12288, static_cast<const int64_t *>(im.ptr), as_stream(stream)));
+    }, py::arg("dst"), py::arg("src"), py::arg("index_map"), py::arg("stream") = 0);
+    m.def("cell_reduce", [](py::object dst, py::object src, py::object index_map, uintptr_t stream) {
+        const char *where = "cell_reduce";
+        Arr d = get_arr(dst, where, "dst"), sr = get_arr(src, where, "src"), im = get_arr(index_map, where, "index_map");
+        require(d, where, "dst", kDLFloat, 32, true, "float32");
+        require(sr, where, "src", kDLFloat, 32, true, "float32");
+        require(im, where, "index_map", kDLInt, 64, true, "int64");
+        if (im.shape.size() != 1 || sr.size != im.size * 12288 || d.size % 12288)
+            raise(std::string(where) + ": expected src of 3*64*64*len(index_map) floats and dst of whole cells");
+        py::gil_scoped_release nogil;
+        check(gmm_cell_reduce(static_cast<float *>(d.ptr), d.size / 12288, static_cast<const float *>(sr.ptr), im.size,
+                              static_cast<const int64_t *>(im.ptr), as_stream(stream)));
+    }, py::arg("dst"), py::arg("src"), py::arg("index_map"), py::arg("stream") = 0);
+    m.def("local_map_to_global", [](const LocalPixelization &lpix, py::object dst, py::object src, uintptr_t stream) {
+        const char *where = "local_map_to_global";
+        Arr d = get_arr(dst, where, "dst"), sr = get_arr(src, where, "src");
+        require(d, where, "dst", kDLFloat, 32, true, "float32");
+        require(sr, where, "src", kDLFloat, 32, true, "float32");
+        const gmm_lpix_info i = lpix.info();
+        if (d.shape.size() != 3 || d.shape[0] != 3 || d.shape[1] != i.nypix_global || d.shape[2] != i.nxpix_global)
+            raise(std::string(where) + ": expected dst of shape (3, nypix_global, nxpix_global)");
+        py::gil_scoped_release nogil;
+        check(gmm_local_map_to_global(lpix.h, static_cast<float *>(d.ptr), static_cast<const float *>(sr.ptr), sr.size,
+                                      as_stream(stream)));
+    }, py::arg("local_pixelization"), py::arg("dst"), py::arg("src"), py::arg("stream") = 0);
     m.def("kernel_timer_enable", [](bool on) { check(gmm_kernel_timer_enable(on ? 1 : 0)); });
     m.def("kernel_timer_read", [](int which) {
         double ms = 0.0;

@@ -329,11 +329,15 @@ class LocalMap:
         are not in the local map are zero. Pixels of boundary cells beyond the global map are
         ignored (no wrapping). Reference: gpu_mm/gpu_mm.py:565-580, src_lib/local_map_to_global.cu.
         """
+        p = self.pixelization
+        if isinstance(self.arr, torch.Tensor) and self.arr.is_cuda and p.cell_offsets_gpu is not None:
+            dst = torch.empty((3, p.nypix_global, p.nxpix_global), dtype=torch.float32, device=self.arr.device)
+            _ext.local_map_to_global(p, dst, self.arr, stream=_stream())
+            return dst.cpu().numpy()
         if isinstance(self.arr, torch.Tensor):
             src = self.arr.detach().to("cpu").numpy().reshape(-1)
         else:
             src = np.asarray(self.arr).reshape(-1)
-        p = self.pixelization
         ny, nx = p.nypix_global, p.nxpix_global
         dst = np.zeros((3, ny, nx), dtype=np.float32)
         off = p.cell_offsets_cpu
@@ -481,6 +485,22 @@ class DynamicMap:
         iy, ix = np.where(self._unstable_pixelization.cell_offsets_cpu >= 0)
         perm = rng.permutation(self.ncells_curr)
         self.permute(iy[perm], ix[perm])
+
+
+def cell_broadcast(dst, src, index_map):
+    """
+    dst cell i = src cell index_map[i], for float32 GPU arrays of whole 3x64x64 cells (flattened)
+    and an int64 GPU index map. Reference: src_lib/cell_broadcast.cu:19-36 (host loop).
+    """
+    _ext.cell_broadcast(dst, src, index_map, stream=_stream())
+
+
+def cell_reduce(dst, src, index_map):
+    """
+    dst cell d = sum of src cells i with index_map[i] == d (zero if none), deterministic.
+    Reference: src_lib/cell_reduce.cu:19-44 (host loop).
+    """
+    _ext.cell_reduce(dst, src, index_map, stream=_stream())
 
 
 def expand_dynamic_map(global_ncells, cell_offsets, plan_mt):

@@ -191,6 +191,28 @@ int gmm_expand_dynamic_map(uint32_t *global_ncells_gpu, int64_t *cell_offsets_gp
 int gmm_lpix_expand_dynamic(gmm_lpix *lpix, const gmm_plan *plan, uint32_t *global_ncells_gpu, uint32_t *new_ncells,
                             void *stream);
 
+/* ---- Cell utilities --------------------------------------------------------------------------
+ * Cells are float[3][64][64] blocks (12288 floats) in cell-list arrays; all pointers are device
+ * pointers, 16-byte aligned; index_map_gpu is int64 on the device. Both raise an error for an
+ * index outside the source/destination range. */
+
+/* cell_broadcast, include/gpu_mm.hpp:251-257; src_lib/cell_broadcast.cu:19-36 (a host loop
+ * there): dst cell i = src cell index_map[i], i < ncells. */
+int gmm_cell_broadcast(float *dst_gpu, int64_t ncells, const float *src_gpu, int64_t nsrc_cells,
+                       const int64_t *index_map_gpu, void *stream);
+
+/* cell_reduce, include/gpu_mm.hpp:259-265; src_lib/cell_reduce.cu:19-44 (a host loop there):
+ * dst cell d = sum over i < nsrc_cells with index_map[i] == d of src cell i (zero if none), summed
+ * in increasing i, so the result is deterministic. */
+int gmm_cell_reduce(float *dst_gpu, int64_t ndst_cells, const float *src_gpu, int64_t nsrc_cells,
+                    const int64_t *index_map_gpu, void *stream);
+
+/* local_map_to_global, include/gpu_mm.hpp:286-291; src_lib/local_map_to_global.cu:11-45 (host
+ * there): dst_gpu float[3][nypix_global][nxpix_global] from the local map (3*npix floats) on the
+ * device; pixels of absent cells are zero. */
+int gmm_local_map_to_global(const gmm_lpix *lpix, float *dst_gpu, const float *src_gpu, int64_t src_size,
+                            void *stream);
+
 /* ---- Kernel timer -------------------------------------------------------------------------
  * No counterpart in the reference; a measurement aid. While enabled, every planned operator
  * records a HIP event pair on its launch stream around its main kernel (planned_map2tod_kernel

@@ -606,3 +606,50 @@ def test_dynamic_map(inst, gmm, orc, dev, masked):
     assert l1rel(got, ref_global) < F64_BAR
     if masked:
         assert np.all(lmap.pixelization.cell_offsets_cpu[~mask] < 0)
+
+
+# ------------------------------------------------------------------------------------------
+# Cell utilities (src_lib/cell_broadcast.cu, cell_reduce.cu, local_map_to_global.cu).
+
+def test_cell_broadcast_reduce(gmm, dev):
+    rng = np.random.default_rng(41)
+    n = 3 * 64 * 64
+    nsrc, nout = 37, 2500
+    src = rng.standard_normal(nsrc * n).astype(F32)
+    idx = rng.integers(0, nsrc, size=nout).astype(np.int64)
+    dst = torch.empty(nout * n, dtype=torch.float32, device=dev)
+    gmm.cell_broadcast(dst, to_gpu(src, dev), to_gpu(idx, dev))
+    assert np.array_equal(to_cpu(dst).reshape(nout, n), src.reshape(nsrc, n)[idx])
+
+    # Reduce back: duplicates accumulate in increasing source order (a sequential float32 sum).
+    red = torch.full((nsrc * n,), 7.0, dtype=torch.float32, device=dev)
+    parts = rng.standard_normal(nout * n).astype(F32)
+    gmm.cell_reduce(red, to_gpu(parts, dev), to_gpu(idx, dev))
+    expect = np.zeros((nsrc, n), dtype=F32)
+    for i in range(nout):
+        expect[idx[i]] += parts.reshape(nout, n)[i]
+    assert np.array_equal(to_cpu(red).reshape(nsrc, n), expect)
+
+    with pytest.raises(RuntimeError, match="out of range"):
+        gmm.cell_broadcast(dst, to_gpu(src, dev), to_gpu(np.full(nout, nsrc, dtype=np.int64), dev))
+    with pytest.raises(RuntimeError, match="out of range"):
+        gmm.cell_reduce(red, to_gpu(parts, dev), to_gpu(np.full(nout, -1, dtype=np.int64), dev))
+
+
+def test_local_map_to_global_gpu(gmm, dev):
+    rng = np.random.default_rng(42)
+    ny, nx = 200, 330
+    cells = [(cy, cx) for cy in range(4) for cx in range(6) if (cy * 7 + cx) % 4 != 1]
+    lp = gmm.LocalPixelization.make_cells(ny, nx, cells, periodic_xcoord=True)
+    arr = rng.standard_normal(3 * lp.npix).astype(F32)
+    expect = gmm.LocalMap(lp, arr).to_global()           # host path (numpy)
+    got = gmm.LocalMap(lp, to_gpu(arr, dev)).to_global()  # device kernel
+    assert got.shape == (3, ny, nx)
+    assert np.array_equal(got, expect)
+    # Independent check of a few pixels.
+    off = lp.cell_offsets_cpu
+    for (y, x) in [(0, 0), (63, 64), (199, 329), (130, 200)]:
+        o = off[y // 64, x // 64]
+        for p in range(3):
+            v = 0.0 if o < 0 else arr[o + p * 4096 + (y % 64) * 64 + (x % 64)]
+            assert got[p, y, x] == v
