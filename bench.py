@@ -30,6 +30,7 @@ if REPO not in sys.path:
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 CELL_BYTES = 3 * 64 * 64 * 4
 MAP_ERR_BAR = 1e-4
+COPY_REF_FLOATS = (1 << 28) + (3 << 20)
 # Algorithmic bytes per sample: 12 B pointing + the TOD element.
 BYTES_PER_SAMPLE = {"f32": 16, "f16": 14, "bf16": 14}
 
@@ -215,7 +216,8 @@ def main():
     ap.add_argument("--nt", type=int, default=262144)
     ap.add_argument("--forms", default="f32,f16,bf16")
     ap.add_argument("--err-ndet", type=int, default=64, help="detectors used for the FP64 error check (0: skip)")
-    ap.add_argument("--cpu-ndet", type=int, default=256)
+    ap.add_argument("--cpu-ndet", type=int, default=1024,
+                    help="detectors in the CPU baseline sample (about 12 s of single-thread work at 1024)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-ref", action="store_true",
                     help="also run a plain 1 GiB device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
@@ -263,7 +265,9 @@ def main():
     chosen = max(ok, key=lambda f: res[f]["gsamp_s"])
 
     if args.copy_ref:
-        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        # A plain device copy of COPY_REF_FLOATS floats (an unusual size, so that its dispatches
+        # are easy to pick out of a counter trace): reads and writes 4*COPY_REF_FLOATS bytes.
+        src = torch.ones(COPY_REF_FLOATS, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
         for _ in range(2):
             dst.copy_(src)
@@ -278,7 +282,7 @@ def main():
     r = res[chosen]
     dom = max(r["ops"], key=lambda k: r["ops"][k]["seconds"])
     traffic = None
-    tfile = os.path.join(REPO, "profiles", f"traffic_{dom}_{chosen}_cfg1.json")
+    tfile = os.path.join(REPO, "profiles", f"traffic_planned_{dom}_{chosen}_{args.ndet}x{args.nt}.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("bytes_per_launch")

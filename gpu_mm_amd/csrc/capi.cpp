@@ -471,14 +471,12 @@ int gmm_preplan_create(gmm_preplan **out, const float *xpointing_gpu, int64_t ns
         int64_t cap = (nmt + 3071) / 3072;
         pp.chunk_cap = std::min<int64_t>(8192, std::max<int64_t>(512, cap));
         pp.max_nchunks = (nmt + pp.chunk_cap - 1) / pp.chunk_cap + std::min(nmt, nkeys);
-        pp.max_nmlines = std::min(pp.pparams.ncl, nmt / 2);
         pp.n_err = std::max<int64_t>(pp.pparams.planner_nblocks, pp.max_nchunks) + 1;
 
         pp.off_err = align128(8 * nmt);
         pp.off_chunks = pp.off_err + align128(4 * pp.n_err);
-        pp.off_mlines = pp.off_chunks + align128(int64_t(sizeof(Chunk)) * pp.max_nchunks);
-        pp.off_absmax = pp.off_mlines + align128(4 * std::max<int64_t>(pp.max_nmlines, 1));
-        pp.plan_nbytes = pp.off_absmax + align128(4 * tod2map_scratch_words());
+        pp.off_qmask = pp.off_chunks + align128(int64_t(sizeof(Chunk)) * pp.max_nchunks);
+        pp.plan_nbytes = pp.off_qmask + align128(nmt);
 
         pp.sort_nbytes = plan_scratch_nbytes(pp) + 128;
         pp.tmp_nbytes = 2 * align128(8 * nmt) + pp.sort_nbytes;
@@ -562,8 +560,7 @@ int gmm_plan_create(gmm_plan **out, const gmm_preplan *pp, const float *xpointin
         plan->plan_mt = reinterpret_cast<uint64_t *>(plan->buf);
         plan->err = reinterpret_cast<uint32_t *>(plan->buf + pp->off_err);
         plan->chunks = reinterpret_cast<Chunk *>(plan->buf + pp->off_chunks);
-        plan->mlines = reinterpret_cast<uint32_t *>(plan->buf + pp->off_mlines);
-        plan->absmax = reinterpret_cast<uint32_t *>(plan->buf + pp->off_absmax);
+        plan->qmask = plan->buf + pp->off_qmask;
 
         try {
             uint8_t *tmp = reinterpret_cast<uint8_t *>(tmp_buf_gpu);
@@ -578,7 +575,8 @@ int gmm_plan_create(gmm_plan **out, const gmm_preplan *pp, const float *xpointin
             launch_plan_emit(xpointing_gpu, *pp, a, plan->err, stream);
             launch_reduce_or(plan->err, pp->pparams.planner_nblocks, plan->err + (pp->n_err - 1), stream);
             plan_sort(*pp, a, b, plan->plan_mt, scratch, scratch_nbytes, stream);
-            plan_schedule(*pp, plan->plan_mt, plan->chunks, plan->mlines, counts, scratch, scratch_nbytes, stream);
+            plan_schedule(*pp, plan->plan_mt, xpointing_gpu, plan->chunks, plan->qmask, counts, scratch, scratch_nbytes,
+                          stream);
 
             uint32_t h[2];
             GMM_HIP(hipMemcpyAsync(h, counts, sizeof(h), hipMemcpyDeviceToHost, stream));
@@ -589,7 +587,7 @@ int gmm_plan_create(gmm_plan **out, const gmm_preplan *pp, const float *xpointin
             }
             plan->nchunks = h[0];
             plan->nmlines = h[1];
-            GMM_REQUIRE(plan->nchunks <= pp->max_nchunks && plan->nmlines <= pp->max_nmlines,
+            GMM_REQUIRE(plan->nchunks <= pp->max_nchunks,
                         "PointingPlan constructor: work list larger than its bound (internal error)");
         }
         catch (...) {

@@ -40,11 +40,10 @@ struct gmm_preplan
     int key_bits = 0;                      // bits of the dense cell key iycell*nxcells_g + ixcell
     int64_t chunk_cap = 0;                 // max plan entries per operator work unit
     int64_t max_nchunks = 0;
-    int64_t max_nmlines = 0;
     int64_t plan_nbytes = 0, tmp_nbytes = 0, sort_nbytes = 0;
 
     // Offsets inside the plan buffer.
-    int64_t off_err = 0, off_chunks = 0, off_mlines = 0, off_absmax = 0, n_err = 0;
+    int64_t off_err = 0, off_chunks = 0, off_qmask = 0, n_err = 0;
 };
 
 struct gmm_plan
@@ -54,8 +53,7 @@ struct gmm_plan
     uint64_t *plan_mt = nullptr;    // sorted plan entries
     uint32_t *err = nullptr;        // per-block error words (n_err of them)
     gmm::Chunk *chunks = nullptr;   // operator work list
-    uint32_t *mlines = nullptr;     // lines that touch more than one cell
-    uint32_t *absmax = nullptr;     // tod2map scratch: per-block max|tod| partials
+    uint8_t *qmask = nullptr;       // per entry: the quads of its line whose samples it owns
     int64_t nchunks = 0;
     int64_t nmlines = 0;
 };
@@ -108,8 +106,8 @@ void plan_sort(const gmm_preplan &pp, uint64_t *a, uint64_t *b, uint64_t *out,
 
 // Builds the chunk list and the multi-cell line list from the sorted plan. Writes the
 // two counts to counts_gpu[0], counts_gpu[1].
-void plan_schedule(const gmm_preplan &pp, const uint64_t *sorted_mt, Chunk *chunks,
-                   uint32_t *mlines, uint32_t *counts_gpu, void *scratch,
+void plan_schedule(const gmm_preplan &pp, const uint64_t *sorted_mt, const float *xp, Chunk *chunks,
+                   uint8_t *qmask, uint32_t *counts_gpu, void *scratch,
                    int64_t scratch_nbytes, hipStream_t stream);
 
 // Size of the scratch used by plan_sort() + plan_schedule().
@@ -185,7 +183,6 @@ void launch_local_to_global(float *dst, const float *src, const gmm_lpix &lp, hi
 void launch_tod16_encode(void *out, int tod_type, float *scale, const float *tod, int64_t nrows,
                          int64_t row_len, hipStream_t stream);
 // Words of tod2map scratch kept in the plan buffer.
-int64_t tod2map_scratch_words();
 
 int64_t unplanned_nblocks(int64_t nsamp, int64_t max_blocks);
 void launch_unplanned_map2tod(float *tod, const float *lmap, const float *xp, const OpArgs &a,

@@ -407,11 +407,11 @@ __device__ inline float m2t_sample(const float *__restrict__ tile, float y, floa
 }
 
 template<bool Periodic, bool Vec, bool HasResp, typename TT>
-__global__ void __launch_bounds__(M2T_THREADS, 6)
+__global__ void __launch_bounds__(M2T_THREADS, 5)
 planned_map2tod_kernel(TT *__restrict__ tod, const float *__restrict__ lmap, const float *__restrict__ xp,
                        const int64_t *__restrict__ cell_offsets, const uint64_t *__restrict__ plan_mt,
-                       const Chunk *__restrict__ chunks, uint32_t *__restrict__ err_word,
-                       int64_t nsamp, int ny, int nx, int nycells, int nxcells,
+                       const Chunk *__restrict__ chunks, const uint8_t *__restrict__ qmask,
+                       uint32_t *__restrict__ err_word, int64_t nsamp, int ny, int nx, int nycells, int nxcells,
                        int64_t ystride, int64_t polstride, bool partial, bool accum, Gains gn)
 {
     __shared__ float tile[3 * HPLANE];
@@ -438,15 +438,18 @@ planned_map2tod_kernel(TT *__restrict__ tod, const float *__restrict__ lmap, con
     for (uint32_t base = wb; base < we; base += 64) {
         const uint32_t cnt = min(64u, we - base);
         const uint32_t packed = (uint32_t(lane) < cnt) ? pack_entry(plan_mt[base + lane]) : 0u;
+        const uint32_t qmw = (uint32_t(lane) < cnt) ? uint32_t(qmask[base + lane]) : 0u;
 
         for (uint32_t j = 0; j < cnt; j += 8) {
             const uint32_t idx = j + li;
             const uint32_t e = __shfl(packed, int(min(idx, 63u)), WAVE);
+            const uint32_t qm = __shfl(qmw, int(min(idx, 63u)), WAVE);
             const uint32_t icl = e & ICL_MASK;
             const bool act = (idx < cnt) && (icl < ncl);
             if (idx < cnt && icl >= ncl)
                 err |= ERR_INCONSISTENT_NMT;
-            if (!act)
+            // Quads without a sample owned by this cell are neither loaded nor evaluated.
+            if (!act || !((qm >> q) & 1u))
                 continue;
             const int64_t s0 = int64_t(icl) * LINE + 4 * q;
             const float4 y4 = load4<Vec>(xp + s0);
@@ -458,8 +461,8 @@ planned_map2tod_kernel(TT *__restrict__ tod, const float *__restrict__ lmap, con
 
             bool o0, o1, o2, o3;
             float4 v;
-            // Scheduling barriers keep the four samples from being interleaved (which would
-            // exceed the register budget of 6 waves per SIMD).
+            // Scheduling barriers keep the four samples from being interleaved, which would
+            // exceed the register budget (96 VGPRs, 5 waves per SIMD).
             v.x = m2t_sample<Periodic, HasResp>(tile, y4.x, x4.x, a4.x, g, ybase, xbase, tr[0], pr[0], chk, o0);
             __builtin_amdgcn_sched_barrier(0);
             v.y = m2t_sample<Periodic, HasResp>(tile, y4.y, x4.y, a4.y, g, ybase, xbase, tr[1], pr[1], chk, o1);
@@ -548,8 +551,9 @@ template<bool Periodic, bool Vec, bool HasResp, typename TT>
 __global__ void __launch_bounds__(T2M_THREADS)
 planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, const float *__restrict__ xp,
                        const int64_t *__restrict__ cell_offsets, const uint64_t *__restrict__ plan_mt,
-                       const Chunk *__restrict__ chunks, uint32_t *__restrict__ err_word, int64_t nsamp, int ny,
-                       int nx, int nycells, int nxcells, int64_t ystride, int64_t polstride, bool partial, Gains gn)
+                       const Chunk *__restrict__ chunks, const uint8_t *__restrict__ qmask,
+                       uint32_t *__restrict__ err_word, int64_t nsamp, int ny, int nx, int nycells, int nxcells,
+                       int64_t ystride, int64_t polstride, bool partial, Gains gn)
 {
     constexpr bool Q16 = !std::is_same<TT, float>::value;
     using IO = TodIO<TT, Vec>;
@@ -628,15 +632,17 @@ planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, con
         for (uint32_t base = wb; base < we; base += 64) {
             const uint32_t cnt = min(64u, we - base);
             const uint32_t packed = (uint32_t(lane) < cnt) ? pack_entry(plan_mt[base + lane]) : 0u;
+            const uint32_t qmw = (uint32_t(lane) < cnt) ? uint32_t(qmask[base + lane]) : 0u;
 
             for (uint32_t j = 0; j < cnt; j += 8) {
                 const uint32_t idx = j + li;
                 const uint32_t e2 = __shfl(packed, int(min(idx, 63u)), WAVE);
+                const uint32_t qm = __shfl(qmw, int(min(idx, 63u)), WAVE);
                 const uint32_t icl = e2 & ICL_MASK;
                 const bool act = (idx < cnt) && (icl < ncl);
                 if (idx < cnt && icl >= ncl)
                     err |= ERR_INCONSISTENT_NMT;
-                if (!act)
+                if (!act || !((qm >> q) & 1u))
                     continue;
                 const int64_t s0 = int64_t(icl) * LINE + 4 * q;
                 const float4 y4 = load4<Vec>(xp + s0);
@@ -767,7 +773,7 @@ void launch_m2t(TT *tod, const float *lmap, const float *xp, const OpArgs &a, co
     auto k = lp.periodic_xcoord ? (vec ? planned_map2tod_kernel<true, true, HasResp, TT> : planned_map2tod_kernel<true, false, HasResp, TT>)
                                 : (vec ? planned_map2tod_kernel<false, true, HasResp, TT> : planned_map2tod_kernel<false, false, HasResp, TT>);
     kernel_timer_mark(TK_MAP2TOD, false, stream);
-    k<<<grid, block, 0, stream>>>(tod, lmap, xp, lp.offsets_gpu, plan.plan_mt, plan.chunks, err_word,
+    k<<<grid, block, 0, stream>>>(tod, lmap, xp, lp.offsets_gpu, plan.plan_mt, plan.chunks, plan.qmask, err_word,
                                   a.nsamp, int(lp.nypix_global), int(lp.nxpix_global), int(lp.nycells),
                                   int(lp.nxcells), lp.ystride, lp.polstride, a.partial, accum, gn);
     GMM_HIP(hipGetLastError());
@@ -785,7 +791,7 @@ void launch_t2m(float *lmap, const TT *tod, const float *xp, const OpArgs &a, co
     auto k = lp.periodic_xcoord ? (vec ? planned_tod2map_kernel<true, true, HasResp, TT> : planned_tod2map_kernel<true, false, HasResp, TT>)
                                 : (vec ? planned_tod2map_kernel<false, true, HasResp, TT> : planned_tod2map_kernel<false, false, HasResp, TT>);
     kernel_timer_mark(TK_TOD2MAP, false, stream);
-    k<<<grid, block, 0, stream>>>(lmap, tod, xp, lp.offsets_gpu, plan.plan_mt, plan.chunks, err_word,
+    k<<<grid, block, 0, stream>>>(lmap, tod, xp, lp.offsets_gpu, plan.plan_mt, plan.chunks, plan.qmask, err_word,
                                   a.nsamp, int(lp.nypix_global), int(lp.nxpix_global), int(lp.nycells),
                                   int(lp.nxcells), lp.ystride, lp.polstride, a.partial, gn);
     GMM_HIP(hipGetLastError());
@@ -844,8 +850,6 @@ void launch_planned_map2tod(void *tod, const float *lmap, const float *xp, const
     default: fail("map2tod: unknown TOD element type");
     }
 }
-
-int64_t tod2map_scratch_words() { return 0; }
 
 void launch_planned_tod2map(float *lmap, const void *tod, const float *xp, const OpArgs &a,
                             const gmm_plan &plan, const TodGains &g, hipStream_t stream)

@@ -535,48 +535,52 @@ __global__ void chunk_fill_kernel(const uint32_t *__restrict__ begin, const uint
     }
 }
 
-// Multi-cell lines: one zflag entry per such line; compacted (stable) into a list of lines.
-__global__ void __launch_bounds__(SC_THREADS)
-zflag_count_kernel(const uint64_t *__restrict__ mt, int64_t n, uint32_t *__restrict__ counts)
+// Owned-quad masks for the operators: bit q of qmask[e] is set iff some sample of quad q
+// (samples 4q..4q+3) of entry e's line has its (iy0, ix0) corner in entry e's cell. A line that
+// lies in one cell owns all 8 quads. Also counts the multi-cell lines (one zflag entry each).
+// Eight consecutive threads handle the eight quads of one entry.
+template<bool Periodic>
+__global__ void __launch_bounds__(256)
+qmask_kernel(const uint64_t *__restrict__ mt, int64_t n, const float *__restrict__ xp, int64_t nsamp, int ny,
+             int nx, uint8_t *__restrict__ qmask, uint32_t *__restrict__ nmlines)
 {
-    __shared__ uint32_t s_wave[SC_THREADS / WAVE];
-    const int64_t t0 = int64_t(blockIdx.x) * SC_TILE + int64_t(threadIdx.x) * SC_ITEMS;
-    uint32_t c = 0;
-#pragma unroll
-    for (int i = 0; i < SC_ITEMS; i++) {
-        int64_t j = t0 + i;
-        c += (j < n && (mt[j] & ZFLAG)) ? 1u : 0u;
-    }
-    uint32_t total;
-    block_exclusive_scan<SC_THREADS>(c, s_wave, total);
+    __shared__ uint32_t s_nml;
     if (threadIdx.x == 0)
-        counts[blockIdx.x] = total;
-}
-
-__global__ void __launch_bounds__(SC_THREADS)
-zflag_compact_kernel(const uint64_t *__restrict__ mt, int64_t n, const uint32_t *__restrict__ offsets,
-                     uint32_t *__restrict__ mlines, int64_t cap)
-{
-    __shared__ uint32_t s_wave[SC_THREADS / WAVE];
-    const int64_t t0 = int64_t(blockIdx.x) * SC_TILE + int64_t(threadIdx.x) * SC_ITEMS;
-    uint64_t v[SC_ITEMS];
-    uint32_t c = 0;
+        s_nml = 0;
+    __syncthreads();
+    const int lane = lane_id();
+    const int64_t total = 8 * n;
+    for (int64_t t0 = int64_t(blockIdx.x) * 256; t0 < total; t0 += int64_t(gridDim.x) * 256) {
+        const int64_t t = t0 + threadIdx.x;
+        const int64_t e = t >> 3;
+        const int q = int(t & 7);
+        bool bit = false, zf = false;
+        if (t < total) {
+            const uint64_t m = mt[e];
+            zf = (q == 0) && (m & ZFLAG);
+            if (!(m & MFLAG))
+                bit = true;
+            else {
+                const uint32_t icell = uint32_t(m) & CELL_MASK;
+                const int64_t s0 = int64_t(uint32_t(m >> ICL_SHIFT) & ICL_MASK) * LINE + 4 * q;
 #pragma unroll
-    for (int i = 0; i < SC_ITEMS; i++) {
-        int64_t j = t0 + i;
-        v[i] = (j < n) ? mt[j] : 0ull;
-        c += (v[i] & ZFLAG) ? 1u : 0u;
-    }
-    uint32_t total;
-    uint32_t pos = block_exclusive_scan<SC_THREADS>(c, s_wave, total) + offsets[blockIdx.x];
-#pragma unroll
-    for (int i = 0; i < SC_ITEMS; i++) {
-        if (v[i] & ZFLAG) {
-            if (int64_t(pos) < cap)
-                mlines[pos] = uint32_t(v[i] >> ICL_SHIFT) & ICL_MASK;
-            pos++;
+                for (int k = 0; k < 4; k++) {
+                    Pixel<float> p;
+                    locate_pixel(xp[s0 + k], xp[nsamp + s0 + k], ny, nx, Periodic, p);
+                    bit |= (cell_key(p.iy0 >> 6, p.ix0 >> 6) == icell);
+                }
+            }
         }
+        const uint64_t b = __ballot(bit);
+        if (t < total && q == 0)
+            qmask[e] = uint8_t((b >> (lane & ~7)) & 0xffu);
+        const uint64_t z = __ballot(zf);
+        if (lane == 0 && z)
+            atomicAdd(&s_nml, uint32_t(__popcll(z)));
     }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_nml)
+        atomicAdd(nmlines, s_nml);
 }
 
 __global__ void copy_word_kernel(const uint32_t *src, uint32_t *dst) { *dst = *src; }
@@ -807,8 +811,8 @@ void build_chunks(const uint64_t *mt, int64_t n, int64_t nkeys, uint32_t kx, uin
 
 }  // namespace
 
-void plan_schedule(const gmm_preplan &pp, const uint64_t *sorted_mt, Chunk *chunks,
-                   uint32_t *mlines, uint32_t *counts_gpu, void *scratch,
+void plan_schedule(const gmm_preplan &pp, const uint64_t *sorted_mt, const float *xp, Chunk *chunks,
+                   uint8_t *qmask, uint32_t *counts_gpu, void *scratch,
                    int64_t scratch_nbytes, hipStream_t stream)
 {
     const int64_t n = pp.plan_nmt;
@@ -820,15 +824,15 @@ void plan_schedule(const gmm_preplan &pp, const uint64_t *sorted_mt, Chunk *chun
     build_chunks(sorted_mt, n, nkeys, uint32_t(pp.nxcells_g), uint32_t(pp.chunk_cap), chunks,
                  pp.max_nchunks, counts_gpu, sc, L, stream);
 
-    // Multi-cell line list.
-    int64_t zt = (n + SC_TILE - 1) / SC_TILE;
-    uint32_t *zc = reinterpret_cast<uint32_t *>(sc + L.zc_off);
-    fill_u32_kernel<<<1, 64, 0, stream>>>(zc + zt, 1, 0u);
-    zflag_count_kernel<<<unsigned(zt), SC_THREADS, 0, stream>>>(sorted_mt, n, zc);
-    GMM_HIP(hipGetLastError());
-    exclusive_scan_u32(zc, zt + 1, sc + L.zc_scan_off, stream);
-    zflag_compact_kernel<<<unsigned(zt), SC_THREADS, 0, stream>>>(sorted_mt, n, zc, mlines, pp.max_nmlines);
-    copy_word_kernel<<<1, 1, 0, stream>>>(zc + zt, counts_gpu + 1);
+    // Owned-quad masks and the multi-cell line count.
+    fill_u32_kernel<<<1, 64, 0, stream>>>(counts_gpu + 1, 1, 0u);
+    const int grid = int(std::min<int64_t>((8 * n + 255) / 256, 2048));
+    if (pp.periodic_xcoord)
+        qmask_kernel<true><<<grid, 256, 0, stream>>>(sorted_mt, n, xp, pp.nsamp, int(pp.nypix_global),
+                                                     int(pp.nxpix_global), qmask, counts_gpu + 1);
+    else
+        qmask_kernel<false><<<grid, 256, 0, stream>>>(sorted_mt, n, xp, pp.nsamp, int(pp.nypix_global),
+                                                      int(pp.nxpix_global), qmask, counts_gpu + 1);
     GMM_HIP(hipGetLastError());
 }
 
