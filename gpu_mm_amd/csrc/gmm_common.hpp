@@ -141,26 +141,9 @@ __host__ __device__ inline uint32_t locate_pixel(T y, T x, int ny, int nx, bool 
 // Larger |x| (not a realistic angle) reduce in double with pi/2 split into three doubles, behind
 // a real branch; beyond |x| ~ 2^52 no float argument carries phase information anyway.
 
-__device__ inline void sincos_angle(float x, float &s, float &c)
+// Polynomials on the reduced argument r (|r| <= pi/4) and the quadrant q.
+__device__ inline void sincos_reduced(float r, int q, float &s, float &c)
 {
-    float r;
-    int q;
-    if (__builtin_expect(fabsf(x) <= 1.0e6f, 1)) {
-        const float j = rintf(x * 0.636619747f);                 // 2/pi
-        r = fmaf(j, -1.57079637f, x);                             // pi/2 = 1.57079637
-        r = fmaf(j, 4.37113883e-08f, r);                          //      - 4.37113883e-8
-        r = fmaf(j, 1.71512451e-15f, r);                          //      - 1.71512451e-15
-        q = int(j);
-    }
-    else {
-        const double xd = double(x);
-        const double jd = rint(xd * 0.6366197723675814);
-        double rd = fma(jd, -1.5707963267948966, xd);
-        rd = fma(jd, -6.123233995736766e-17, rd);
-        rd = fma(jd, 1.4973849048591698e-33, rd);
-        r = float(rd);
-        q = int(jd - 4.0 * floor(jd * 0.25));
-    }
     const float z = r * r;
     const float sp = fmaf(r * z, fmaf(z, fmaf(z, -1.95139321e-04f, 8.33214819e-03f), -1.66666538e-01f), r);
     const float cp = fmaf(z * z, fmaf(z, fmaf(z, 2.44305738e-05f, -1.38872897e-03f), 4.16666456e-02f),
@@ -169,6 +152,30 @@ __device__ inline void sincos_angle(float x, float &s, float &c)
     const float cc = (q & 1) ? sp : cp;
     s = (q & 2) ? -ss : ss;
     c = ((q + 1) & 2) ? -cc : cc;
+}
+
+// |x| <= 1e6 only (the float Cody-Waite reduction).
+__device__ inline void sincos_angle_small(float x, float &s, float &c)
+{
+    const float j = rintf(x * 0.636619747f);                 // 2/pi
+    float r = fmaf(j, -1.57079637f, x);                       // pi/2 = 1.57079637
+    r = fmaf(j, 4.37113883e-08f, r);                          //      - 4.37113883e-8
+    r = fmaf(j, 1.71512451e-15f, r);                          //      - 1.71512451e-15
+    sincos_reduced(r, int(j), s, c);
+}
+
+__device__ inline void sincos_angle(float x, float &s, float &c)
+{
+    if (__builtin_expect(fabsf(x) <= 1.0e6f, 1)) {
+        sincos_angle_small(x, s, c);
+        return;
+    }
+    const double xd = double(x);
+    const double jd = rint(xd * 0.6366197723675814);
+    double rd = fma(jd, -1.5707963267948966, xd);
+    rd = fma(jd, -6.123233995736766e-17, rd);
+    rd = fma(jd, 1.4973849048591698e-33, rd);
+    sincos_reduced(float(rd), int(jd - 4.0 * floor(jd * 0.25)), s, c);
 }
 
 // The (up to four) distinct cells touched by the bilinear footprint of one sample, as

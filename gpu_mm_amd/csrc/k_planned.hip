@@ -234,14 +234,6 @@ __device__ inline void quad_scale(const Gains &g, uint32_t s0, float (&sc)[4])
 //   nan_y, nan_x = sums of the coordinates (NaN iff some coordinate is NaN, or infinite ones
 //   of both signs, which the range check flags anyway).
 
-__device__ inline int cvt_rz(float v)
-{
-    // v_cvt_i32_f32 rounds toward zero, saturates and maps NaN to 0 (as trunc_to_int).
-    int r;
-    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(v));
-    return r;
-}
-
 struct Checks
 {
     float y = -INFINITY, x = -INFINITY, ny = 0.0f, nx = 0.0f;
@@ -263,6 +255,7 @@ struct Geom
     float ymax;     // ny - 1
     float xlo, xhi; // valid x range: [0, nx-2] or, periodic, [-nx, 2nx]
     float fnx;
+    float fny2, fnx2; // ny - 2, nx - 2
 };
 
 __device__ inline Geom make_geom(int ny, int nx, bool periodic)
@@ -272,6 +265,8 @@ __device__ inline Geom make_geom(int ny, int nx, bool periodic)
     g.nx = nx;
     g.ymax = float(ny - 1);
     g.fnx = float(nx);
+    g.fny2 = float(ny - 2);
+    g.fnx2 = float(nx - 2);
     g.xlo = periodic ? -float(nx) : 0.0f;
     g.xhi = periodic ? float(2 * nx) : float(nx - 2);
     return g;
@@ -285,20 +280,22 @@ __device__ inline void locate_op(float y, float x, const Geom &g, Checks &c, int
     c.ny += y;
     c.nx += x;
 
-    int iy = cvt_rz(y);
-    iy = min(max(iy, 0), g.ny - 2);
+    // Clamping in float before the conversion gives the same index as truncating and then
+    // clamping (NaN goes to 0 through fmaxf) for every in-range coordinate; the conversion is
+    // then always defined. Out-of-range coordinates are flagged above.
+    const int iy = int(fminf(fmaxf(y, 0.0f), g.fny2));
     dy = y - float(iy);
     iy0 = iy;
 
     if (Periodic) {
         x = (x >= 0.0f) ? x : (x + g.fnx);
         x = (x <= g.fnx) ? x : (x - g.fnx);
-        int ix = max(cvt_rz(x), 0);
+        const int ix = int(fminf(fmaxf(x, 0.0f), g.fnx));
         dx = x - float(ix);
         ix0 = (ix < g.nx) ? ix : 0;
     }
     else {
-        int ix = min(max(cvt_rz(x), 0), g.nx - 2);
+        const int ix = int(fminf(fmaxf(x, 0.0f), g.fnx2));
         dx = x - float(ix);
         ix0 = ix;
     }
@@ -391,7 +388,24 @@ __device__ inline float m2t_interp(const float *__restrict__ tile, int o, float 
     return fmaf(dy, bot - top, top);
 }
 
-template<bool Periodic, bool HasResp>
+// Small: every angle of the wave's current quads satisfies |2 alpha| <= 1e6 (checked once per
+// quad for the whole wave), so the float reduction suffices and no per-sample branch is needed.
+template<bool Small>
+__device__ inline void sincos2(float alpha, float &sn, float &cs)
+{
+    if (Small)
+        sincos_angle_small(2.0f * alpha, sn, cs);
+    else
+        sincos_angle(2.0f * alpha, sn, cs);
+}
+
+__device__ inline bool quad_small_angles(const float4 &a4)
+{
+    const float m = fmaxf(fmaxf(fabsf(a4.x), fabsf(a4.y)), fmaxf(fabsf(a4.z), fabsf(a4.w)));
+    return !__any(!(m <= 5.0e5f));
+}
+
+template<bool Periodic, bool HasResp, bool Small>
 __device__ inline float m2t_sample(const float *__restrict__ tile, float y, float x, float alpha, const Geom &g,
                                    int ybase, int xbase, float tr, float pr, Checks &chk, bool &owned)
 {
@@ -401,7 +415,7 @@ __device__ inline float m2t_sample(const float *__restrict__ tile, float y, floa
     const int ly = iy0 - ybase, lx = ix0 - xbase;
     owned = (unsigned(ly) < 64u) && (unsigned(lx) < 64u);
     float sn, cs;
-    sincos_angle(2.0f * alpha, sn, cs);
+    sincos2<Small>(alpha, sn, cs);
     const int o = owned ? (ly * HS + lx) : 0;
     return m2t_interp<HasResp>(tile, o, dy, dx, cs, sn, tr, pr);
 }
@@ -462,16 +476,21 @@ planned_map2tod_kernel(TT *__restrict__ tod, const float *__restrict__ lmap, con
             bool o0, o1, o2, o3;
             float4 v;
             // Scheduling barriers keep the four samples from being interleaved, which would
-            // exceed the register budget (96 VGPRs, 5 waves per SIMD).
-            v.x = m2t_sample<Periodic, HasResp>(tile, y4.x, x4.x, a4.x, g, ybase, xbase, tr[0], pr[0], chk, o0);
+            // exceed the register budget (96 VGPRs, 5 waves per SIMD). (A wave-uniform split into
+            // a small-angle path, as in tod2map, costs this kernel registers and spills.)
+            v.x = m2t_sample<Periodic, HasResp, false>(tile, y4.x, x4.x, a4.x, g, ybase, xbase, tr[0], pr[0], chk, o0);
             __builtin_amdgcn_sched_barrier(0);
-            v.y = m2t_sample<Periodic, HasResp>(tile, y4.y, x4.y, a4.y, g, ybase, xbase, tr[1], pr[1], chk, o1);
+            v.y = m2t_sample<Periodic, HasResp, false>(tile, y4.y, x4.y, a4.y, g, ybase, xbase, tr[1], pr[1], chk, o1);
             __builtin_amdgcn_sched_barrier(0);
-            v.z = m2t_sample<Periodic, HasResp>(tile, y4.z, x4.z, a4.z, g, ybase, xbase, tr[2], pr[2], chk, o2);
+            v.z = m2t_sample<Periodic, HasResp, false>(tile, y4.z, x4.z, a4.z, g, ybase, xbase, tr[2], pr[2], chk, o2);
             __builtin_amdgcn_sched_barrier(0);
-            v.w = m2t_sample<Periodic, HasResp>(tile, y4.w, x4.w, a4.w, g, ybase, xbase, tr[3], pr[3], chk, o3);
+            v.w = m2t_sample<Periodic, HasResp, false>(tile, y4.w, x4.w, a4.w, g, ybase, xbase, tr[3], pr[3], chk, o3);
             if constexpr (!std::is_same<TT, float>::value) {
-                v.x /= sc[0]; v.y /= sc[1]; v.z /= sc[2]; v.w /= sc[3];
+                // The hardware reciprocal (1 ulp) is exact enough ahead of rounding to 16 bits.
+                v.x *= __builtin_amdgcn_rcpf(sc[0]);
+                v.y *= __builtin_amdgcn_rcpf(sc[1]);
+                v.z *= __builtin_amdgcn_rcpf(sc[2]);
+                v.w *= __builtin_amdgcn_rcpf(sc[3]);
             }
             TT *t = tod + s0;
             if (o0 && o1 && o2 && o3) {
@@ -499,7 +518,7 @@ planned_map2tod_kernel(TT *__restrict__ tod, const float *__restrict__ lmap, con
 // Fixed-point deposit of w*v (already scaled) into a 64-bit LDS accumulator.
 __device__ inline void dep_fixed(unsigned long long *__restrict__ a, float v)
 {
-    const int iv = cvt_rz(v);
+    const int iv = int(v);      // |v| < 2^28 by the choice of scale
     atomicAdd(a, (unsigned long long)(long long)iv);
 }
 
@@ -529,7 +548,7 @@ __device__ inline void deposit4(unsigned long long *__restrict__ acc, float *__r
 
 // One sample of tod2map: locate; if the block owns it (its (iy0, ix0) corner lies in the
 // block's cell), deposit all four corners into the halo tile.
-template<bool Periodic, bool HasResp, bool Fixed>
+template<bool Periodic, bool HasResp, bool Fixed, bool Small>
 __device__ inline void t2m_sample(unsigned long long *__restrict__ acc, float *__restrict__ facc, float y, float x,
                                   float alpha, float t, const Geom &g, int ybase, int xbase, float tr, float pr,
                                   Checks &chk)
@@ -540,7 +559,7 @@ __device__ inline void t2m_sample(unsigned long long *__restrict__ acc, float *_
     const int ly = iy0 - ybase, lx = ix0 - xbase;
     if ((unsigned(ly) < 64u) && (unsigned(lx) < 64u)) {
         float sn, cs;
-        sincos_angle(2.0f * alpha, sn, cs);
+        sincos2<Small>(alpha, sn, cs);
         const float ts = HasResp ? t * tr : t;
         const float ps = HasResp ? t * pr : t;
         deposit4<Fixed>(acc, facc, ly * HS + lx, dy, dx, ts, ps * cs, ps * sn);
@@ -673,16 +692,24 @@ planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, con
                     t4.y *= scale;
                     t4.z *= scale;
                     t4.w *= scale;
-                    t2m_sample<Periodic, HasResp, true>(acc, facc, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, tr[0], pr[0], chk);
-                    t2m_sample<Periodic, HasResp, true>(acc, facc, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, tr[1], pr[1], chk);
-                    t2m_sample<Periodic, HasResp, true>(acc, facc, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, tr[2], pr[2], chk);
-                    t2m_sample<Periodic, HasResp, true>(acc, facc, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, tr[3], pr[3], chk);
+                    if (quad_small_angles(a4)) {
+                        t2m_sample<Periodic, HasResp, true, true>(acc, facc, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, tr[0], pr[0], chk);
+                        t2m_sample<Periodic, HasResp, true, true>(acc, facc, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, tr[1], pr[1], chk);
+                        t2m_sample<Periodic, HasResp, true, true>(acc, facc, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, tr[2], pr[2], chk);
+                        t2m_sample<Periodic, HasResp, true, true>(acc, facc, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, tr[3], pr[3], chk);
+                    }
+                    else {
+                        t2m_sample<Periodic, HasResp, true, false>(acc, facc, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, tr[0], pr[0], chk);
+                        t2m_sample<Periodic, HasResp, true, false>(acc, facc, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, tr[1], pr[1], chk);
+                        t2m_sample<Periodic, HasResp, true, false>(acc, facc, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, tr[2], pr[2], chk);
+                        t2m_sample<Periodic, HasResp, true, false>(acc, facc, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, tr[3], pr[3], chk);
+                    }
                 }
                 else {
-                    t2m_sample<Periodic, HasResp, false>(acc, facc, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, tr[0], pr[0], chk);
-                    t2m_sample<Periodic, HasResp, false>(acc, facc, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, tr[1], pr[1], chk);
-                    t2m_sample<Periodic, HasResp, false>(acc, facc, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, tr[2], pr[2], chk);
-                    t2m_sample<Periodic, HasResp, false>(acc, facc, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, tr[3], pr[3], chk);
+                    t2m_sample<Periodic, HasResp, false, false>(acc, facc, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, tr[0], pr[0], chk);
+                    t2m_sample<Periodic, HasResp, false, false>(acc, facc, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, tr[1], pr[1], chk);
+                    t2m_sample<Periodic, HasResp, false, false>(acc, facc, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, tr[2], pr[2], chk);
+                    t2m_sample<Periodic, HasResp, false, false>(acc, facc, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, tr[3], pr[3], chk);
                 }
             }
         }
