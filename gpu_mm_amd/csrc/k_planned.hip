@@ -388,26 +388,62 @@ __device__ inline float m2t_interp(const float *__restrict__ tile, int o, float 
     return fmaf(dy, bot - top, top);
 }
 
-// Small: every angle of the wave's current quads satisfies |2 alpha| <= 1e6 (checked once per
-// quad for the whole wave), so the float reduction suffices and no per-sample branch is needed.
-template<bool Small>
-__device__ inline void sincos2(float alpha, float &sn, float &cs)
+// ------------------------------------------------------------------------------------------
+// sin and cos of 2*alpha from a 256-entry table in LDS: x = k * (2 pi / 256) + d with
+// k = rint(x * 256 / (2 pi)) and d from a three-float Cody-Waite split of 2 pi / 256, then
+// sin x = S_k cos d + C_k sin d, cos x = C_k cos d - S_k sin d with cos d = 1 - d^2/2 and
+// sin d = d (1 - d^2/6). For |x| <= 2e5 the product x * 256/(2 pi) is below 2^23, so k is within
+// 0.75 of the exact quotient and |d| <= 0.0185, where the dropped terms (d^4/24 < 5e-9,
+// d^5/120 < 2e-11) are below float rounding. Larger |x| use sincos_angle(). The table holds the
+// float-rounded values of the double-precision sin/cos of 2 pi i / 256.
+
+constexpr int SCT = 256;
+
+__device__ inline void fill_sincos_table(float2 *__restrict__ tab, int nthreads)
 {
-    if (Small)
-        sincos_angle_small(2.0f * alpha, sn, cs);
+    for (int i = threadIdx.x; i < SCT; i += nthreads) {
+        double sd, cd;
+        sincos(double(i) * (6.283185307179586 / SCT), &sd, &cd);
+        tab[i] = make_float2(float(sd), float(cd));
+    }
+}
+
+__device__ inline void sincos_tab(const float2 *__restrict__ tab, float x, float &s, float &c)
+{
+    const float k = rintf(x * 40.7436638f);              // 256 / (2 pi)
+    float d = fmaf(k, -0.024543693f, x);                  // 2 pi / 256 = 0.024543693
+    d = fmaf(k, 6.8299044e-10f, d);                       //            - 6.8299044e-10
+    d = fmaf(k, 2.7755576e-17f, d);                       //            - 2.7755576e-17
+    const float2 t = tab[int(k) & (SCT - 1)];
+    const float z = d * d;
+    const float cd = fmaf(z, -0.5f, 1.0f);
+    const float sd = d * fmaf(z, -0.16666667f, 1.0f);
+    s = fmaf(t.x, cd, t.y * sd);
+    c = fmaf(t.y, cd, -(t.x * sd));
+}
+
+// Small: every angle of the wave's current quads satisfies |2 alpha| <= 2e5 (checked once per
+// quad for the whole wave), so the table path needs no per-sample branch.
+template<bool Small>
+__device__ inline void sincos2(const float2 *__restrict__ tab, float alpha, float &sn, float &cs)
+{
+    const float x = 2.0f * alpha;
+    if (Small || fabsf(x) <= 2.0e5f)
+        sincos_tab(tab, x, sn, cs);
     else
-        sincos_angle(2.0f * alpha, sn, cs);
+        sincos_angle(x, sn, cs);
 }
 
 __device__ inline bool quad_small_angles(const float4 &a4)
 {
     const float m = fmaxf(fmaxf(fabsf(a4.x), fabsf(a4.y)), fmaxf(fabsf(a4.z), fabsf(a4.w)));
-    return !__any(!(m <= 5.0e5f));
+    return !__any(!(m <= 1.0e5f));
 }
 
 template<bool Periodic, bool HasResp, bool Small>
-__device__ inline float m2t_sample(const float *__restrict__ tile, float y, float x, float alpha, const Geom &g,
-                                   int ybase, int xbase, float tr, float pr, Checks &chk, bool &owned)
+__device__ inline float m2t_sample(const float *__restrict__ tile, const float2 *__restrict__ tab, float y, float x,
+                                   float alpha, const Geom &g, int ybase, int xbase, float tr, float pr, Checks &chk,
+                                   bool &owned)
 {
     int iy0, ix0;
     float dy, dx;
@@ -415,7 +451,7 @@ __device__ inline float m2t_sample(const float *__restrict__ tile, float y, floa
     const int ly = iy0 - ybase, lx = ix0 - xbase;
     owned = (unsigned(ly) < 64u) && (unsigned(lx) < 64u);
     float sn, cs;
-    sincos2<Small>(alpha, sn, cs);
+    sincos2<Small>(tab, alpha, sn, cs);
     const int o = owned ? (ly * HS + lx) : 0;
     return m2t_interp<HasResp>(tile, o, dy, dx, cs, sn, tr, pr);
 }
@@ -429,12 +465,14 @@ planned_map2tod_kernel(TT *__restrict__ tod, const float *__restrict__ lmap, con
                        int64_t ystride, int64_t polstride, bool partial, bool accum, Gains gn)
 {
     __shared__ float tile[3 * HPLANE];
+    __shared__ float2 sct[SCT];
 
     const Chunk ch = chunks[blockIdx.x];
     const HaloCells h = halo_cells(ch.icell, cell_offsets, nycells, nxcells, nx, Periodic);
     const uint32_t ncl = uint32_t(nsamp >> 5);
     uint32_t err = (h.own < 0 && !partial) ? ERR_NOT_IN_PIXELIZATION : 0u;
 
+    fill_sincos_table(sct, M2T_THREADS);
     stage_halo_tile<M2T_THREADS>(tile, lmap, h, ystride, polstride);
     __syncthreads();
 
@@ -478,13 +516,13 @@ planned_map2tod_kernel(TT *__restrict__ tod, const float *__restrict__ lmap, con
             // Scheduling barriers keep the four samples from being interleaved, which would
             // exceed the register budget (96 VGPRs, 5 waves per SIMD). (A wave-uniform split into
             // a small-angle path, as in tod2map, costs this kernel registers and spills.)
-            v.x = m2t_sample<Periodic, HasResp, false>(tile, y4.x, x4.x, a4.x, g, ybase, xbase, tr[0], pr[0], chk, o0);
+            v.x = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.x, x4.x, a4.x, g, ybase, xbase, tr[0], pr[0], chk, o0);
             __builtin_amdgcn_sched_barrier(0);
-            v.y = m2t_sample<Periodic, HasResp, false>(tile, y4.y, x4.y, a4.y, g, ybase, xbase, tr[1], pr[1], chk, o1);
+            v.y = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.y, x4.y, a4.y, g, ybase, xbase, tr[1], pr[1], chk, o1);
             __builtin_amdgcn_sched_barrier(0);
-            v.z = m2t_sample<Periodic, HasResp, false>(tile, y4.z, x4.z, a4.z, g, ybase, xbase, tr[2], pr[2], chk, o2);
+            v.z = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.z, x4.z, a4.z, g, ybase, xbase, tr[2], pr[2], chk, o2);
             __builtin_amdgcn_sched_barrier(0);
-            v.w = m2t_sample<Periodic, HasResp, false>(tile, y4.w, x4.w, a4.w, g, ybase, xbase, tr[3], pr[3], chk, o3);
+            v.w = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.w, x4.w, a4.w, g, ybase, xbase, tr[3], pr[3], chk, o3);
             if constexpr (!std::is_same<TT, float>::value) {
                 // The hardware reciprocal (1 ulp) is exact enough ahead of rounding to 16 bits.
                 v.x *= __builtin_amdgcn_rcpf(sc[0]);
@@ -549,9 +587,9 @@ __device__ inline void deposit4(unsigned long long *__restrict__ acc, float *__r
 // One sample of tod2map: locate; if the block owns it (its (iy0, ix0) corner lies in the
 // block's cell), deposit all four corners into the halo tile.
 template<bool Periodic, bool HasResp, bool Fixed, bool Small>
-__device__ inline void t2m_sample(unsigned long long *__restrict__ acc, float *__restrict__ facc, float y, float x,
-                                  float alpha, float t, const Geom &g, int ybase, int xbase, float tr, float pr,
-                                  Checks &chk)
+__device__ inline void t2m_sample(unsigned long long *__restrict__ acc, float *__restrict__ facc,
+                                  const float2 *__restrict__ tab, float y, float x, float alpha, float t, const Geom &g,
+                                  int ybase, int xbase, float tr, float pr, Checks &chk)
 {
     int iy0, ix0;
     float dy, dx;
@@ -559,7 +597,7 @@ __device__ inline void t2m_sample(unsigned long long *__restrict__ acc, float *_
     const int ly = iy0 - ybase, lx = ix0 - xbase;
     if ((unsigned(ly) < 64u) && (unsigned(lx) < 64u)) {
         float sn, cs;
-        sincos2<Small>(alpha, sn, cs);
+        sincos2<Small>(tab, alpha, sn, cs);
         const float ts = HasResp ? t * tr : t;
         const float ps = HasResp ? t * pr : t;
         deposit4<Fixed>(acc, facc, ly * HS + lx, dy, dx, ts, ps * cs, ps * sn);
@@ -577,6 +615,7 @@ planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, con
     constexpr bool Q16 = !std::is_same<TT, float>::value;
     using IO = TodIO<TT, Vec>;
     __shared__ unsigned long long acc[3 * HPLANE];       // 99 KiB
+    __shared__ float2 sct[SCT];
     __shared__ uint32_t s_max, s_redo;
     float *facc = reinterpret_cast<float *>(acc);         // float tile for the non-finite fallback
 
@@ -607,6 +646,7 @@ planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, con
         s_max = 0;
         s_redo = 0;
     }
+    fill_sincos_table(sct, T2M_THREADS);
     __syncthreads();
     {
         float m = 0.0f;
@@ -693,23 +733,23 @@ planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, con
                     t4.z *= scale;
                     t4.w *= scale;
                     if (quad_small_angles(a4)) {
-                        t2m_sample<Periodic, HasResp, true, true>(acc, facc, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, tr[0], pr[0], chk);
-                        t2m_sample<Periodic, HasResp, true, true>(acc, facc, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, tr[1], pr[1], chk);
-                        t2m_sample<Periodic, HasResp, true, true>(acc, facc, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, tr[2], pr[2], chk);
-                        t2m_sample<Periodic, HasResp, true, true>(acc, facc, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, tr[3], pr[3], chk);
+                        t2m_sample<Periodic, HasResp, true, true>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, tr[0], pr[0], chk);
+                        t2m_sample<Periodic, HasResp, true, true>(acc, facc, sct, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, tr[1], pr[1], chk);
+                        t2m_sample<Periodic, HasResp, true, true>(acc, facc, sct, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, tr[2], pr[2], chk);
+                        t2m_sample<Periodic, HasResp, true, true>(acc, facc, sct, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, tr[3], pr[3], chk);
                     }
                     else {
-                        t2m_sample<Periodic, HasResp, true, false>(acc, facc, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, tr[0], pr[0], chk);
-                        t2m_sample<Periodic, HasResp, true, false>(acc, facc, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, tr[1], pr[1], chk);
-                        t2m_sample<Periodic, HasResp, true, false>(acc, facc, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, tr[2], pr[2], chk);
-                        t2m_sample<Periodic, HasResp, true, false>(acc, facc, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, tr[3], pr[3], chk);
+                        t2m_sample<Periodic, HasResp, true, false>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, tr[0], pr[0], chk);
+                        t2m_sample<Periodic, HasResp, true, false>(acc, facc, sct, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, tr[1], pr[1], chk);
+                        t2m_sample<Periodic, HasResp, true, false>(acc, facc, sct, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, tr[2], pr[2], chk);
+                        t2m_sample<Periodic, HasResp, true, false>(acc, facc, sct, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, tr[3], pr[3], chk);
                     }
                 }
                 else {
-                    t2m_sample<Periodic, HasResp, false, false>(acc, facc, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, tr[0], pr[0], chk);
-                    t2m_sample<Periodic, HasResp, false, false>(acc, facc, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, tr[1], pr[1], chk);
-                    t2m_sample<Periodic, HasResp, false, false>(acc, facc, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, tr[2], pr[2], chk);
-                    t2m_sample<Periodic, HasResp, false, false>(acc, facc, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, tr[3], pr[3], chk);
+                    t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, tr[0], pr[0], chk);
+                    t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, tr[1], pr[1], chk);
+                    t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, tr[2], pr[2], chk);
+                    t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, tr[3], pr[3], chk);
                 }
             }
         }
