@@ -33,6 +33,7 @@ HIP_FLAGS = [
 
 LIB_SOURCES = [
     ("capi.cpp", ["-x", "hip"]),
+    ("capi_batch.cpp", ["-x", "hip"]),
     ("host_ref.cpp", ["-x", "hip", "-ffp-contract=off"]),
     ("k_planner.hip", []),
     ("k_planned.hip", []),

@@ -95,32 +95,30 @@ void check_npix_global(int64_t n, const char *what, const char *where)
     }
 }
 
-namespace {
-
-template<typename F>
-int guarded(F &&f)
-{
-    try {
-        f();
-        return 0;
-    }
-    catch (const std::exception &e) {
-        g_last_error = e.what();
-        return 1;
-    }
-    catch (...) {
-        g_last_error = "unknown C++ exception";
-        return 1;
-    }
-}
-
-inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+void set_last_error(const char *msg) { g_last_error = msg; }
 
 void require_ptr(const void *p, const char *where, const char *name)
 {
     if (!p)
         fail(std::string(where) + ": '" + name + "' is a null pointer");
 }
+
+uint32_t read_word(const uint32_t *p, hipStream_t stream)
+{
+    uint32_t h = 0;
+    GMM_HIP(hipMemcpyAsync(&h, p, 4, hipMemcpyDeviceToHost, stream));
+    GMM_HIP(hipStreamSynchronize(stream));
+    return h;
+}
+
+void clear_word(uint32_t *p, hipStream_t stream)
+{
+    static const uint32_t zero = 0;
+    GMM_HIP(hipMemcpyAsync(p, &zero, 4, hipMemcpyHostToDevice, stream));
+    GMM_HIP(hipStreamSynchronize(stream));
+}
+
+namespace {
 
 void init_npix(gmm_lpix &lp, const char *where)
 {
@@ -140,21 +138,6 @@ void init_npix(gmm_lpix &lp, const char *where)
     max_offset += 2 * lp.polstride + 63 * lp.ystride + 63;
     if (lp.npix > 0 && max_offset != 3 * lp.npix - 1)
         fail(std::string(where) + ": something is wrong with the cell_offsets -- the cells do not fit together contiguously");
-}
-
-uint32_t read_word(const uint32_t *p, hipStream_t stream)
-{
-    uint32_t h = 0;
-    GMM_HIP(hipMemcpyAsync(&h, p, 4, hipMemcpyDeviceToHost, stream));
-    GMM_HIP(hipStreamSynchronize(stream));
-    return h;
-}
-
-void clear_word(uint32_t *p, hipStream_t stream)
-{
-    static const uint32_t zero = 0;
-    GMM_HIP(hipMemcpyAsync(p, &zero, 4, hipMemcpyHostToDevice, stream));
-    GMM_HIP(hipStreamSynchronize(stream));
 }
 
 // Synchronous check of the plan's accumulated error word; clears it when set.

@@ -232,4 +232,33 @@ struct Chunk
 
 constexpr uint32_t CHUNK_SHARED = 1u;
 
+// Batched operators (several TODs with their own plans, one local map). Device-resident tables.
+struct BatchTod
+{
+    const float *xp;              // xpointing [3][nsamp]
+    void *tod;                    // TOD [nsamp] (float, or 16-bit with per-row scales)
+    const float *scale;           // 16-bit TODs: float[nsamp / scale_nper], else null
+    const uint64_t *plan_mt;      // the TOD's sorted plan entries
+    const uint8_t *qmask;         // the plan's owned-quad masks
+    int64_t nsamp;
+    uint32_t scale_nper;          // samples per scale row
+    uint32_t pad;
+};
+
+struct BatchSeg
+{
+    uint32_t tod;                 // index into the BatchTod table
+    uint32_t begin, end;          // plan entries [begin, end) of that TOD, all of one cell
+    uint32_t pad;
+};
+
+struct BatchItem
+{
+    uint32_t icell;               // cell key (iycell << 10 | ixcell)
+    uint32_t seg_begin, seg_end;  // segments [seg_begin, seg_end)
+    uint32_t nent;                // total plan entries of the segments
+    uint32_t flags;               // CHUNK_SHARED: the cell's entries are split over several items
+    uint32_t pad[3];
+};
+
 }  // namespace gmm

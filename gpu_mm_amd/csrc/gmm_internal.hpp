@@ -71,6 +71,33 @@ void hip_check(hipError_t e, const char *what);
 #define GMM_HIP(call) ::gmm::hip_check((call), #call)
 #define GMM_REQUIRE(cond, msg) do { if (!(cond)) ::gmm::fail(std::string(msg)); } while (0)
 
+// C-ABI plumbing (capi.cpp): every entry point runs its body through guarded(), which turns an
+// exception into status 1 and the thread's last-error message.
+void set_last_error(const char *msg);
+
+template<typename F>
+int guarded(F &&f)
+{
+    try {
+        f();
+        return 0;
+    }
+    catch (const std::exception &e) {
+        set_last_error(e.what());
+        return 1;
+    }
+    catch (...) {
+        set_last_error("unknown C++ exception");
+        return 1;
+    }
+}
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+void require_ptr(const void *p, const char *where, const char *name);
+// Synchronous read / clear of one device word on 'stream'.
+uint32_t read_word(const uint32_t *p, hipStream_t stream);
+void clear_word(uint32_t *p, hipStream_t stream);
+
 // Converts a set of error bits into the exception the reference raises
 // (src_lib/check_arguments.cu:159-175).
 void throw_errflags(uint32_t err, const char *where, uint32_t ignore = 0);
@@ -164,6 +191,27 @@ void launch_planned_map2tod(void *tod, const float *lmap, const float *xp, const
                             const gmm_plan &plan, bool accum, const TodGains &g, hipStream_t stream);
 void launch_planned_tod2map(float *lmap, const void *tod, const float *xp, const OpArgs &a,
                             const gmm_plan &plan, const TodGains &g, hipStream_t stream);
+// Batched operators (k_planned.hip): device tables of TODs, segments and work items, the queue
+// counters (2 words, zero between launches) and the error word. grid > 0: persistent grid of
+// that many blocks taking items from the queue; grid == 0: one block per item. threads: block
+// size of map2tod (256 or 512); tod2map always uses 1024.
+struct BatchLaunch
+{
+    const BatchTod *tods = nullptr;
+    const BatchSeg *segs = nullptr;
+    const BatchItem *items = nullptr;
+    int64_t nitems = 0;
+    uint32_t *counters = nullptr;
+    uint32_t *err_word = nullptr;
+    int64_t grid = 0;
+    int threads = 512;
+};
+
+void launch_batched_map2tod(const BatchLaunch &bl, const float *lmap, const gmm_lpix &lp, int tod_type, bool partial,
+                            bool accum, hipStream_t stream);
+void launch_batched_tod2map(const BatchLaunch &bl, float *lmap, const gmm_lpix &lp, int tod_type, bool partial,
+                            hipStream_t stream);
+
 // Dynamic map expansion (cells of a cell-sorted plan_mt into cell_offsets, new indices in
 // cell-key order); scratch of expand_dynamic_scratch_nbytes(nmt) bytes.
 int64_t expand_dynamic_scratch_nbytes(int64_t nmt);

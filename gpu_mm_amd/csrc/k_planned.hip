@@ -337,10 +337,10 @@ __device__ inline HaloCells halo_cells(uint32_t icell, const int64_t *__restrict
 
 template<int NT>
 __device__ inline void stage_halo_tile(float *__restrict__ tile, const float *__restrict__ lmap, const HaloCells &h,
-                                       int64_t ystride, int64_t polstride)
+                                       int64_t ystride, int64_t polstride, int tid)
 {
     // Interior (column W excluded when W < 64: it holds the wrapped column).
-    for (int i = threadIdx.x; i < 3 * CELL_NPIX; i += NT) {
+    for (int i = tid; i < 3 * CELL_NPIX; i += NT) {
         const int p = i >> 12, r = (i >> 6) & 63, c = i & 63;
         if (c == h.w)
             continue;
@@ -348,7 +348,7 @@ __device__ inline void stage_halo_tile(float *__restrict__ tile, const float *__
     }
     // Row 64 (columns 0..W) and column W (rows 0..63).
     const int nrow = h.w + 1, nper = nrow + 64;
-    for (int i = threadIdx.x; i < 3 * nper; i += NT) {
+    for (int i = tid; i < 3 * nper; i += NT) {
         const int p = i / nper, j = i - p * nper;
         int64_t src;
         int idx;
@@ -456,37 +456,21 @@ __device__ inline float m2t_sample(const float *__restrict__ tile, const float2 
     return m2t_interp<HasResp>(tile, o, dy, dx, cs, sn, tr, pr);
 }
 
+// map2tod of plan entries [wb, we) of one TOD by the calling wave, with the halo tile of the
+// block's cell (whose top-left pixel is (ybase, xbase)) in LDS. Every sample is written once, by
+// the block that owns it: no atomics and no zeroing.
 template<bool Periodic, bool Vec, bool HasResp, typename TT>
-__global__ void __launch_bounds__(M2T_THREADS, 5)
-planned_map2tod_kernel(TT *__restrict__ tod, const float *__restrict__ lmap, const float *__restrict__ xp,
-                       const int64_t *__restrict__ cell_offsets, const uint64_t *__restrict__ plan_mt,
-                       const Chunk *__restrict__ chunks, const uint8_t *__restrict__ qmask,
-                       uint32_t *__restrict__ err_word, int64_t nsamp, int ny, int nx, int nycells, int nxcells,
-                       int64_t ystride, int64_t polstride, bool partial, bool accum, Gains gn)
+__device__ inline void m2t_entries(TT *__restrict__ tod, const float *__restrict__ xp,
+                                   const uint64_t *__restrict__ plan_mt, const uint8_t *__restrict__ qmask,
+                                   int64_t nsamp, uint32_t wb, uint32_t we, const float *__restrict__ tile,
+                                   const float2 *__restrict__ sct, const Geom &g, int ybase, int xbase, bool accum,
+                                   const Gains &gn, Checks &chk, uint32_t &err)
 {
-    __shared__ float tile[3 * HPLANE];
-    __shared__ float2 sct[SCT];
-
-    const Chunk ch = chunks[blockIdx.x];
-    const HaloCells h = halo_cells(ch.icell, cell_offsets, nycells, nxcells, nx, Periodic);
-    const uint32_t ncl = uint32_t(nsamp >> 5);
-    uint32_t err = (h.own < 0 && !partial) ? ERR_NOT_IN_PIXELIZATION : 0u;
-
-    fill_sincos_table(sct, M2T_THREADS);
-    stage_halo_tile<M2T_THREADS>(tile, lmap, h, ystride, polstride);
-    __syncthreads();
-
-    const Geom g = make_geom(ny, nx, Periodic);
-    Checks chk;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
     const int li = lane >> 3, q = lane & 7;
-    const int ybase = int(ch.icell >> 10) << 6, xbase = int(ch.icell & 1023u) << 6;
+    const uint32_t ncl = uint32_t(nsamp >> 5);
     using IO = TodIO<TT, Vec>;
 
-    uint32_t wb, we;
-    wave_slice(ch, wave, M2T_THREADS / WAVE, wb, we);
-
-    // Every sample is written once, by the block that owns it: no atomics and no zeroing.
     for (uint32_t base = wb; base < we; base += 64) {
         const uint32_t cnt = min(64u, we - base);
         const uint32_t packed = (uint32_t(lane) < cnt) ? pack_entry(plan_mt[base + lane]) : 0u;
@@ -547,7 +531,36 @@ planned_map2tod_kernel(TT *__restrict__ tod, const float *__restrict__ lmap, con
             }
         }
     }
+}
 
+template<bool Periodic, bool Vec, bool HasResp, typename TT>
+__global__ void __launch_bounds__(M2T_THREADS, 5)
+planned_map2tod_kernel(TT *__restrict__ tod, const float *__restrict__ lmap, const float *__restrict__ xp,
+                       const int64_t *__restrict__ cell_offsets, const uint64_t *__restrict__ plan_mt,
+                       const Chunk *__restrict__ chunks, const uint8_t *__restrict__ qmask,
+                       uint32_t *__restrict__ err_word, int64_t nsamp, int ny, int nx, int nycells, int nxcells,
+                       int64_t ystride, int64_t polstride, bool partial, bool accum, Gains gn)
+{
+    __shared__ float tile[3 * HPLANE];
+    __shared__ float2 sct[SCT];
+
+    const Chunk ch = chunks[blockIdx.x];
+    const HaloCells h = halo_cells(ch.icell, cell_offsets, nycells, nxcells, nx, Periodic);
+    uint32_t err = (h.own < 0 && !partial) ? ERR_NOT_IN_PIXELIZATION : 0u;
+
+    fill_sincos_table(sct, M2T_THREADS);
+    stage_halo_tile<M2T_THREADS>(tile, lmap, h, ystride, polstride, threadIdx.x);
+    __syncthreads();
+
+    const Geom g = make_geom(ny, nx, Periodic);
+    Checks chk;
+    const int wave = threadIdx.x >> 6;
+    const int ybase = int(ch.icell >> 10) << 6, xbase = int(ch.icell & 1023u) << 6;
+
+    uint32_t wb, we;
+    wave_slice(ch, wave, M2T_THREADS / WAVE, wb, we);
+    m2t_entries<Periodic, Vec, HasResp, TT>(tod, xp, plan_mt, qmask, nsamp, wb, we, tile, sct, g, ybase, xbase, accum,
+                                            gn, chk, err);
     report_err(err | chk.bits(), err_word);
 }
 
@@ -604,191 +617,153 @@ __device__ inline void t2m_sample(unsigned long long *__restrict__ acc, float *_
     }
 }
 
-template<bool Periodic, bool Vec, bool HasResp, typename TT>
-__global__ void __launch_bounds__(T2M_THREADS)
-planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, const float *__restrict__ xp,
-                       const int64_t *__restrict__ cell_offsets, const uint64_t *__restrict__ plan_mt,
-                       const Chunk *__restrict__ chunks, const uint8_t *__restrict__ qmask,
-                       uint32_t *__restrict__ err_word, int64_t nsamp, int ny, int nx, int nycells, int nxcells,
-                       int64_t ystride, int64_t polstride, bool partial, Gains gn)
+// Largest |t| (times the gains) over the first 8 plan entries of [wb, we), for the calling
+// lane (lane 8*li + q reads quad q of entry wb + li).
+template<bool Vec, bool HasResp, typename TT>
+__device__ inline float t2m_probe(const TT *__restrict__ tod, const uint64_t *__restrict__ plan_mt, int64_t nsamp,
+                                  uint32_t wb, uint32_t we, const Gains &gn)
 {
     constexpr bool Q16 = !std::is_same<TT, float>::value;
     using IO = TodIO<TT, Vec>;
-    __shared__ unsigned long long acc[3 * HPLANE];       // 99 KiB
-    __shared__ float2 sct[SCT];
-    __shared__ uint32_t s_max, s_redo;
-    float *facc = reinterpret_cast<float *>(acc);         // float tile for the non-finite fallback
-
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const Chunk ch = chunks[blockIdx.x];
-    const HaloCells h = halo_cells(ch.icell, cell_offsets, nycells, nxcells, nx, Periodic);
-    const uint32_t ncl = uint32_t(nsamp >> 5);
-    uint32_t err = (h.own < 0 && !partial) ? ERR_NOT_IN_PIXELIZATION : 0u;
-
-    // Samples owned by an absent cell still deposit into present neighbours through the halo,
-    // so every block runs; cells that are absent receive nothing (tod2map_reference.cu:14-46
-    // skips them too).
-
-    const Geom g = make_geom(ny, nx, Periodic);
+    const int lane = threadIdx.x & 63;
     const int li = lane >> 3, q = lane & 7;
-    const int ybase = int(ch.icell >> 10) << 6, xbase = int(ch.icell & 1023u) << 6;
-    uint32_t wb, we;
-    wave_slice(ch, wave, T2M_THREADS / WAVE, wb, we);
-
-    // Fixed-point scale. Deposits are w * t * S with S = 2^(27 - e) and |t| < 2^(e+1), so each
-    // converts exactly to int32 (|.| < 2^28) and the tile sums in int64. The exponent e comes
-    // from an estimate: 4 x the largest |t| (times the gains) among the first 8 lines of every
-    // wave's slice. Each lane tracks the largest |t| it deposits; if any exceeds the estimate's
-    // range, or is not finite, the block discards its tile and runs its chunk again, with the
-    // exact maximum, or with float atomics when that maximum is not finite. No pass over the
-    // TOD is needed before the kernel.
-    if (threadIdx.x == 0) {
-        s_max = 0;
-        s_redo = 0;
-    }
-    fill_sincos_table(sct, T2M_THREADS);
-    __syncthreads();
-    {
-        float m = 0.0f;
-        const uint32_t idx = wb + uint32_t(li);
-        if (idx < we) {
-            const uint32_t icl = uint32_t(plan_mt[idx] >> ICL_SHIFT) & ICL_MASK;
-            if (icl < ncl) {
-                const int64_t s0 = int64_t(icl) * LINE + 4 * q;
-                const float4 t4 = IO::load(tod + s0);
-                float tr[4], pr[4], sc[4];
-                quad_resp<HasResp>(gn, uint32_t(s0), tr, pr);
-                quad_scale<TT>(gn, uint32_t(s0), sc);
-                const float tv[4] = { t4.x, t4.y, t4.z, t4.w };
+    const uint32_t ncl = uint32_t(nsamp >> 5);
+    float m = 0.0f;
+    const uint32_t idx = wb + uint32_t(li);
+    if (idx < we) {
+        const uint32_t icl = uint32_t(plan_mt[idx] >> ICL_SHIFT) & ICL_MASK;
+        if (icl < ncl) {
+            const int64_t s0 = int64_t(icl) * LINE + 4 * q;
+            const float4 t4 = IO::load(tod + s0);
+            float tr[4], pr[4], sc[4];
+            quad_resp<HasResp>(gn, uint32_t(s0), tr, pr);
+            quad_scale<TT>(gn, uint32_t(s0), sc);
+            const float tv[4] = { t4.x, t4.y, t4.z, t4.w };
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const float t = Q16 ? tv[k] * sc[k] : tv[k];
-                    m = fmaxf(m, fabsf(HasResp ? t * tr[k] : t));
-                    m = fmaxf(m, fabsf(HasResp ? t * pr[k] : t));
-                }
+            for (int k = 0; k < 4; k++) {
+                const float t = Q16 ? tv[k] * sc[k] : tv[k];
+                m = fmaxf(m, fabsf(HasResp ? t * tr[k] : t));
+                m = fmaxf(m, fabsf(HasResp ? t * pr[k] : t));
             }
         }
-        const uint32_t mw = wave_max(__float_as_uint(m));
-        if (lane == 0)
-            atomicMax(&s_max, mw);
     }
-    for (int i = threadIdx.x; i < 3 * HPLANE; i += T2M_THREADS)
-        acc[i] = 0ull;
-    __syncthreads();
-    uint32_t mbits = s_max;
-    mbits = (mbits < FINITE_LIMIT) ? __float_as_uint(4.0f * __uint_as_float(mbits)) : mbits;
-    bool finite = (mbits < FINITE_LIMIT);
+    return m;
+}
 
-    for (;;) {
-        const int e = int(mbits >> 23) - 127;            // binary exponent of the bound
-        const int sexp = clamp_i(27 - e, -100, 126);
-        const float scale = __uint_as_float(uint32_t(sexp + 127) << 23);
-        const float limit = __uint_as_float(uint32_t(e + 128) << 23);   // 2^(e+1)
-        float mrun = 0.0f;                               // largest |t| deposited by this lane
-        float tsum = 0.0f;                               // NaN iff some deposited t is NaN
-        Checks chk;
+// tod2map of plan entries [wb, we) of one TOD by the calling wave into the block's halo tile:
+// fixed point with scale 'scale' when 'finite', else float atomics. Tracks per lane the largest
+// |t| deposited (mrun) and a sum that is NaN iff some deposited t is NaN (tsum).
+template<bool Periodic, bool Vec, bool HasResp, typename TT>
+__device__ inline void t2m_entries(unsigned long long *__restrict__ acc, float *__restrict__ facc,
+                                   const TT *__restrict__ tod, const float *__restrict__ xp,
+                                   const uint64_t *__restrict__ plan_mt, const uint8_t *__restrict__ qmask,
+                                   int64_t nsamp, uint32_t wb, uint32_t we, const float2 *__restrict__ sct,
+                                   const Geom &g, int ybase, int xbase, const Gains &gn, bool finite, float scale,
+                                   float &mrun, float &tsum, Checks &chk, uint32_t &err)
+{
+    constexpr bool Q16 = !std::is_same<TT, float>::value;
+    using IO = TodIO<TT, Vec>;
+    const int lane = threadIdx.x & 63;
+    const int li = lane >> 3, q = lane & 7;
+    const uint32_t ncl = uint32_t(nsamp >> 5);
 
-        for (uint32_t base = wb; base < we; base += 64) {
-            const uint32_t cnt = min(64u, we - base);
-            const uint32_t packed = (uint32_t(lane) < cnt) ? pack_entry(plan_mt[base + lane]) : 0u;
-            const uint32_t qmw = (uint32_t(lane) < cnt) ? uint32_t(qmask[base + lane]) : 0u;
+    for (uint32_t base = wb; base < we; base += 64) {
+        const uint32_t cnt = min(64u, we - base);
+        const uint32_t packed = (uint32_t(lane) < cnt) ? pack_entry(plan_mt[base + lane]) : 0u;
+        const uint32_t qmw = (uint32_t(lane) < cnt) ? uint32_t(qmask[base + lane]) : 0u;
 
-            for (uint32_t j = 0; j < cnt; j += 8) {
-                const uint32_t idx = j + li;
-                const uint32_t e2 = __shfl(packed, int(min(idx, 63u)), WAVE);
-                const uint32_t qm = __shfl(qmw, int(min(idx, 63u)), WAVE);
-                const uint32_t icl = e2 & ICL_MASK;
-                const bool act = (idx < cnt) && (icl < ncl);
-                if (idx < cnt && icl >= ncl)
-                    err |= ERR_INCONSISTENT_NMT;
-                if (!act || !((qm >> q) & 1u))
-                    continue;
-                const int64_t s0 = int64_t(icl) * LINE + 4 * q;
-                const float4 y4 = load4<Vec>(xp + s0);
-                const float4 x4 = load4<Vec>(xp + nsamp + s0);
-                const float4 a4 = load4<Vec>(xp + 2 * nsamp + s0);
-                float4 t4 = IO::load(tod + s0);
-                float tr[4], pr[4], sc[4];
-                quad_resp<HasResp>(gn, uint32_t(s0), tr, pr);
-                quad_scale<TT>(gn, uint32_t(s0), sc);
-                if (Q16) {
-                    t4.x *= sc[0];
-                    t4.y *= sc[1];
-                    t4.z *= sc[2];
-                    t4.w *= sc[3];
+        for (uint32_t j = 0; j < cnt; j += 8) {
+            const uint32_t idx = j + li;
+            const uint32_t e2 = __shfl(packed, int(min(idx, 63u)), WAVE);
+            const uint32_t qm = __shfl(qmw, int(min(idx, 63u)), WAVE);
+            const uint32_t icl = e2 & ICL_MASK;
+            const bool act = (idx < cnt) && (icl < ncl);
+            if (idx < cnt && icl >= ncl)
+                err |= ERR_INCONSISTENT_NMT;
+            if (!act || !((qm >> q) & 1u))
+                continue;
+            const int64_t s0 = int64_t(icl) * LINE + 4 * q;
+            const float4 y4 = load4<Vec>(xp + s0);
+            const float4 x4 = load4<Vec>(xp + nsamp + s0);
+            const float4 a4 = load4<Vec>(xp + 2 * nsamp + s0);
+            float4 t4 = IO::load(tod + s0);
+            float tr[4], pr[4], sc[4];
+            quad_resp<HasResp>(gn, uint32_t(s0), tr, pr);
+            quad_scale<TT>(gn, uint32_t(s0), sc);
+            if (Q16) {
+                t4.x *= sc[0];
+                t4.y *= sc[1];
+                t4.z *= sc[2];
+                t4.w *= sc[3];
+            }
+            if (finite) {
+                // NaN makes fmaxf skip the value: count it through the sum instead.
+                if (HasResp) {
+                    mrun = fmaxf(mrun, fmaxf(fabsf(t4.x * tr[0]), fabsf(t4.x * pr[0])));
+                    mrun = fmaxf(mrun, fmaxf(fabsf(t4.y * tr[1]), fabsf(t4.y * pr[1])));
+                    mrun = fmaxf(mrun, fmaxf(fabsf(t4.z * tr[2]), fabsf(t4.z * pr[2])));
+                    mrun = fmaxf(mrun, fmaxf(fabsf(t4.w * tr[3]), fabsf(t4.w * pr[3])));
                 }
-                if (finite) {
-                    // NaN makes fmaxf skip the value: count it through the sum instead.
-                    if (HasResp) {
-                        mrun = fmaxf(mrun, fmaxf(fabsf(t4.x * tr[0]), fabsf(t4.x * pr[0])));
-                        mrun = fmaxf(mrun, fmaxf(fabsf(t4.y * tr[1]), fabsf(t4.y * pr[1])));
-                        mrun = fmaxf(mrun, fmaxf(fabsf(t4.z * tr[2]), fabsf(t4.z * pr[2])));
-                        mrun = fmaxf(mrun, fmaxf(fabsf(t4.w * tr[3]), fabsf(t4.w * pr[3])));
-                    }
-                    else
-                        mrun = fmaxf(mrun, fmaxf(fmaxf(fabsf(t4.x), fabsf(t4.y)), fmaxf(fabsf(t4.z), fabsf(t4.w))));
-                    tsum += (t4.x + t4.y) + (t4.z + t4.w);
-                    t4.x *= scale;
-                    t4.y *= scale;
-                    t4.z *= scale;
-                    t4.w *= scale;
-                    if (quad_small_angles(a4)) {
-                        t2m_sample<Periodic, HasResp, true, true>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, tr[0], pr[0], chk);
-                        t2m_sample<Periodic, HasResp, true, true>(acc, facc, sct, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, tr[1], pr[1], chk);
-                        t2m_sample<Periodic, HasResp, true, true>(acc, facc, sct, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, tr[2], pr[2], chk);
-                        t2m_sample<Periodic, HasResp, true, true>(acc, facc, sct, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, tr[3], pr[3], chk);
-                    }
-                    else {
-                        t2m_sample<Periodic, HasResp, true, false>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, tr[0], pr[0], chk);
-                        t2m_sample<Periodic, HasResp, true, false>(acc, facc, sct, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, tr[1], pr[1], chk);
-                        t2m_sample<Periodic, HasResp, true, false>(acc, facc, sct, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, tr[2], pr[2], chk);
-                        t2m_sample<Periodic, HasResp, true, false>(acc, facc, sct, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, tr[3], pr[3], chk);
-                    }
+                else
+                    mrun = fmaxf(mrun, fmaxf(fmaxf(fabsf(t4.x), fabsf(t4.y)), fmaxf(fabsf(t4.z), fabsf(t4.w))));
+                tsum += (t4.x + t4.y) + (t4.z + t4.w);
+                t4.x *= scale;
+                t4.y *= scale;
+                t4.z *= scale;
+                t4.w *= scale;
+                if (quad_small_angles(a4)) {
+                    t2m_sample<Periodic, HasResp, true, true>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, tr[0], pr[0], chk);
+                    t2m_sample<Periodic, HasResp, true, true>(acc, facc, sct, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, tr[1], pr[1], chk);
+                    t2m_sample<Periodic, HasResp, true, true>(acc, facc, sct, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, tr[2], pr[2], chk);
+                    t2m_sample<Periodic, HasResp, true, true>(acc, facc, sct, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, tr[3], pr[3], chk);
                 }
                 else {
-                    t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, tr[0], pr[0], chk);
-                    t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, tr[1], pr[1], chk);
-                    t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, tr[2], pr[2], chk);
-                    t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, tr[3], pr[3], chk);
+                    t2m_sample<Periodic, HasResp, true, false>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, tr[0], pr[0], chk);
+                    t2m_sample<Periodic, HasResp, true, false>(acc, facc, sct, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, tr[1], pr[1], chk);
+                    t2m_sample<Periodic, HasResp, true, false>(acc, facc, sct, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, tr[2], pr[2], chk);
+                    t2m_sample<Periodic, HasResp, true, false>(acc, facc, sct, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, tr[3], pr[3], chk);
                 }
             }
+            else {
+                t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, tr[0], pr[0], chk);
+                t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, tr[1], pr[1], chk);
+                t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, tr[2], pr[2], chk);
+                t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, tr[3], pr[3], chk);
+            }
         }
-        err |= chk.bits();
-        const bool lane_nan = !(tsum == tsum);
-        if (!finite)
-            break;
-        // Any value beyond the estimate's range or non-finite: redo the chunk.
-        const uint32_t mw = wave_max(lane_nan ? FINITE_LIMIT : __float_as_uint(mrun));
-        if (lane == 0 && mw >= __float_as_uint(limit))
-            atomicMax(&s_redo, mw);
-        __syncthreads();
-        const uint32_t redo = s_redo;
-        if (redo == 0)
-            break;
-        // Redo with the exact maximum (now within range), or in floating point.
-        mbits = redo;
-        finite = (mbits < FINITE_LIMIT);
-        for (int i = threadIdx.x; i < 3 * HPLANE; i += T2M_THREADS) {
-            if (finite)
-                acc[i] = 0ull;
-            else if (i < (3 * HPLANE + 1) / 2)
-                acc[i] = 0ull;        // the float tile is the first half
-        }
-        __syncthreads();
-        if (threadIdx.x == 0)
-            s_redo = 0;
     }
-    __syncthreads();
-    const int sexp_f = clamp_i(27 - (int(mbits >> 23) - 127), -100, 126);
-    const float inv_scale = __uint_as_float(uint32_t(127 - sexp_f) << 23);
+}
 
-    // Flush. The cell's row 0 and column 0 also receive other blocks' halo deposits (and column
-    // 0 of cell column 0 the wrapped ones), so they and every split cell are added atomically;
-    // the rest of the cell with a plain read-modify-write. Halo row 64 and column W go to the
-    // neighbouring cells, atomically; deposits into absent cells are dropped.
-    const bool shared = (ch.flags & CHUNK_SHARED) != 0;
+// Fixed-point scale exponent for a bound with float bits 'mbits': S = 2^(27 - e).
+__device__ inline int t2m_scale_exp(uint32_t mbits)
+{
+    return clamp_i(27 - (int(mbits >> 23) - 127), -100, 126);
+}
+
+// Clears the halo tile: all of it for the fixed-point form, the float view's part otherwise.
+template<int NT>
+__device__ inline void t2m_clear(unsigned long long *__restrict__ acc, bool finite, int tid)
+{
+    for (int i = tid; i < 3 * HPLANE; i += NT) {
+        if (finite)
+            acc[i] = 0ull;
+        else if (i < (3 * HPLANE + 1) / 2)
+            acc[i] = 0ull;        // the float tile is the first half
+    }
+}
+
+// Flush of the halo tile into the map. The cell's row 0 and column 0 also receive other
+// blocks' halo deposits (and column 0 of cell column 0 the wrapped ones), so they and every
+// split cell are added atomically; the rest of the cell with a plain read-modify-write. Halo row
+// 64 and column W go to the neighbouring cells, atomically; deposits into absent cells are
+// dropped.
+template<int NT>
+__device__ inline void t2m_flush(float *__restrict__ lmap, const unsigned long long *__restrict__ acc,
+                                 const float *__restrict__ facc, const HaloCells &h, bool shared, bool finite,
+                                 float inv_scale, int64_t ystride, int64_t polstride, int tid)
+{
     const int nrow = h.w + 1, nper = nrow + 64;
-    for (int i = threadIdx.x; i < 3 * (CELL_NPIX + nper); i += T2M_THREADS) {
+    for (int i = tid; i < 3 * (CELL_NPIX + nper); i += NT) {
         int pl, k;
         int64_t dst;
         bool atomic = true;
@@ -825,8 +800,317 @@ planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, con
         else
             *gp += v;
     }
+}
 
+// Deposits of one block's work unit into its halo tile, in fixed point with redo rounds.
+// Before the call the tile is clear (fixed-point form), *s_max holds the float bits of the
+// block's probe maximum, s_redo[0..2] are zero, and the block has synchronized.
+// run(finite, scale, mrun, tsum, chk) deposits the calling wave's samples (t2m_entries()).
+// Deposits are w * t * S with S = 2^(27 - e) and |t| < 2^(e+1), so each converts exactly to
+// int32 (|.| < 2^28) and the tile sums in int64. The exponent e comes from an estimate: 4 x the
+// probe maximum. Each lane tracks the largest |t| it deposits; if any exceeds the estimate's
+// range, or is not finite, the block discards its tile and runs again, with the exact maximum
+// (which cannot trigger another round), or with float atomics when that maximum is not finite:
+// at most three rounds, each with its own redo word. Returns whether the tile is fixed point
+// and its inverse scale; ends synchronized.
+template<int NT, typename Run>
+__device__ inline bool t2m_accumulate(unsigned long long *__restrict__ acc, const uint32_t *s_max, uint32_t *s_redo,
+                                      Run &&run, float &inv_scale, uint32_t &err, int tid)
+{
+    const int lane = threadIdx.x & 63;
+    uint32_t mbits = *s_max;
+    mbits = (mbits < FINITE_LIMIT) ? __float_as_uint(4.0f * __uint_as_float(mbits)) : mbits;
+    bool finite = (mbits < FINITE_LIMIT);
+
+    for (int pass = 0;; pass++) {
+        const int e = int(mbits >> 23) - 127;            // binary exponent of the bound
+        const float scale = __uint_as_float(uint32_t(t2m_scale_exp(mbits) + 127) << 23);
+        const float limit = __uint_as_float(uint32_t(e + 128) << 23);   // 2^(e+1)
+        float mrun = 0.0f;                               // largest |t| deposited by this lane
+        float tsum = 0.0f;                               // NaN iff some deposited t is NaN
+        Checks chk;
+        run(finite, scale, mrun, tsum, chk);
+        err |= chk.bits();
+        if (!finite)
+            break;
+        const bool lane_nan = !(tsum == tsum);
+        uint32_t *redo_word = s_redo + min(pass, 2);
+        // Any value beyond the estimate's range or non-finite: redo.
+        const uint32_t mw = wave_max(lane_nan ? FINITE_LIMIT : __float_as_uint(mrun));
+        if (lane == 0 && mw >= __float_as_uint(limit))
+            atomicMax(redo_word, mw);
+        __syncthreads();
+        const uint32_t redo = *redo_word;
+        if (redo == 0)
+            break;
+        // Redo with the exact maximum (now within range), or in floating point.
+        mbits = redo;
+        finite = (mbits < FINITE_LIMIT);
+        t2m_clear<NT>(acc, finite, tid);
+        __syncthreads();
+    }
+    __syncthreads();
+    inv_scale = __uint_as_float(uint32_t(127 - t2m_scale_exp(mbits)) << 23);
+    return finite;
+}
+
+template<bool Periodic, bool Vec, bool HasResp, typename TT>
+__global__ void __launch_bounds__(T2M_THREADS)
+planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, const float *__restrict__ xp,
+                       const int64_t *__restrict__ cell_offsets, const uint64_t *__restrict__ plan_mt,
+                       const Chunk *__restrict__ chunks, const uint8_t *__restrict__ qmask,
+                       uint32_t *__restrict__ err_word, int64_t nsamp, int ny, int nx, int nycells, int nxcells,
+                       int64_t ystride, int64_t polstride, bool partial, Gains gn)
+{
+    __shared__ unsigned long long acc[3 * HPLANE];       // 99 KiB
+    __shared__ float2 sct[SCT];
+    __shared__ uint32_t s_max, s_redo[3];
+    float *facc = reinterpret_cast<float *>(acc);         // float tile for the non-finite fallback
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const Chunk ch = chunks[blockIdx.x];
+    const HaloCells h = halo_cells(ch.icell, cell_offsets, nycells, nxcells, nx, Periodic);
+    uint32_t err = (h.own < 0 && !partial) ? ERR_NOT_IN_PIXELIZATION : 0u;
+
+    // Samples owned by an absent cell still deposit into present neighbours through the halo,
+    // so every block runs; cells that are absent receive nothing (tod2map_reference.cu:14-46
+    // skips them too).
+
+    const Geom g = make_geom(ny, nx, Periodic);
+    const int ybase = int(ch.icell >> 10) << 6, xbase = int(ch.icell & 1023u) << 6;
+    uint32_t wb, we;
+    wave_slice(ch, wave, T2M_THREADS / WAVE, wb, we);
+
+    // The fixed-point bound is estimated in the kernel from the first lines of every wave's
+    // slice (t2m_accumulate() describes the scheme): no pass over the TOD is needed first.
+    if (threadIdx.x < 4) {
+        if (threadIdx.x == 0)
+            s_max = 0;
+        else
+            s_redo[threadIdx.x - 1] = 0;
+    }
+    fill_sincos_table(sct, T2M_THREADS);
+    __syncthreads();
+    {
+        const float m = t2m_probe<Vec, HasResp, TT>(tod, plan_mt, nsamp, wb, we, gn);
+        const uint32_t mw = wave_max(__float_as_uint(m));
+        if (lane == 0)
+            atomicMax(&s_max, mw);
+    }
+    t2m_clear<T2M_THREADS>(acc, true, threadIdx.x);
+    __syncthreads();
+
+    float inv_scale;
+    const bool finite = t2m_accumulate<T2M_THREADS>(
+        acc, &s_max, s_redo,
+        [&](bool fin, float scale, float &mrun, float &tsum, Checks &chk) {
+            t2m_entries<Periodic, Vec, HasResp, TT>(acc, facc, tod, xp, plan_mt, qmask, nsamp, wb, we, sct, g, ybase,
+                                                    xbase, gn, fin, scale, mrun, tsum, chk, err);
+        },
+        inv_scale, err, threadIdx.x);
+    t2m_flush<T2M_THREADS>(lmap, acc, facc, h, (ch.flags & CHUNK_SHARED) != 0, finite, inv_scale, ystride,
+                           polstride, threadIdx.x);
     report_err(err, err_word);
+}
+
+// ------------------------------------------------------------------------------------------
+// Batched operators: several TODs, each with its own plan, against one local map, in one launch.
+// A work item is a run of plan entries of one cell, gathered from the plans of several TODs as
+// a list of segments (BatchSeg: entries [begin, end) of TOD 'tod'). The block of an item stages
+// (map2tod) or accumulates (tod2map) the cell's halo tile once for all of its segments; its
+// waves split the item's entries evenly, across segment boundaries. With 'persistent', a grid
+// of a few blocks per CU takes items from a queue counter, in the host's order (largest first);
+// the last block to finish resets the counters, so a launch needs no host-side reset and can be
+// captured in a graph. Otherwise there is one block per item.
+
+// The tables themselves are separate __restrict__ kernel parameters, which lets the compiler
+// read them through the scalar unit (wave-uniform values in scalar registers).
+struct BatchArgs
+{
+    uint32_t nitems;
+    uint32_t persistent;
+    uint32_t *counters;            // 2 words: next item, finished blocks
+    uint32_t *err_word;
+    int ny, nx, nycells, nxcells;
+    int64_t ystride, polstride;
+    bool partial, accum;
+};
+
+// Item of the block's next work unit, the same in every lane: the next one from the queue
+// (persistent grid) or the block's own. Ends synchronized. (Other threads may reset
+// block-shared words before the call: the synchronization covers them.)
+__device__ inline uint32_t batch_next_item(const BatchArgs &ba, uint32_t *s_item)
+{
+    if (!ba.persistent)
+        return blockIdx.x;
+    if (threadIdx.x == 0)
+        *s_item = atomicAdd(ba.counters, 1u);
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(*s_item);
+}
+
+// threadIdx.x, opaque to the compiler, so that the thread-dependent index arithmetic of the
+// per-item staging and flush loops is not hoisted out of the item loop, where it would hold
+// registers across the whole item.
+__device__ inline int opaque_tid()
+{
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
+__device__ inline void batch_finish(const BatchArgs &ba)
+{
+    if (ba.persistent && threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(ba.counters + 1, 1u) == gridDim.x - 1) {
+            // Every block has taken its last item: the queue can be rewound for the next launch.
+            atomicExch(ba.counters, 0u);
+            atomicExch(ba.counters + 1, 0u);
+        }
+    }
+}
+
+// Calls f(seg, begin, end) for the parts of the item's segments inside the wave's entry slice
+// [wb, we) (item-relative entry indices).
+template<typename F>
+__device__ inline void batch_wave_segments(const BatchSeg *__restrict__ segs, const BatchItem &it, uint32_t wb,
+                                           uint32_t we, F &&f)
+{
+    uint32_t pos = 0;
+    for (uint32_t s = it.seg_begin; s < it.seg_end && pos < we; s++) {
+        const BatchSeg sg = segs[s];
+        const uint32_t len = sg.end - sg.begin;
+        const uint32_t lo = max(wb, pos), hi = min(we, pos + len);
+        if (lo < hi)
+            f(sg, sg.begin + (lo - pos), sg.begin + (hi - pos));
+        pos += len;
+    }
+}
+
+__device__ inline Gains batch_gains(const BatchTod &td)
+{
+    return Gains{ nullptr, td.scale, 1u, 1u, td.scale_nper };
+}
+
+template<bool Periodic, typename TT, int NT>
+__global__ void __launch_bounds__(NT, 5)
+batched_map2tod_kernel(const float *__restrict__ lmap, const BatchTod *__restrict__ tods,
+                       const BatchSeg *__restrict__ segs, const BatchItem *__restrict__ items,
+                       const int64_t *__restrict__ cell_offsets, BatchArgs ba)
+{
+    __shared__ float tile[3 * HPLANE];
+    __shared__ float2 sct[SCT];
+    __shared__ uint32_t s_item;
+
+    fill_sincos_table(sct, NT);
+    const Geom g = make_geom(ba.ny, ba.nx, Periodic);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    Checks chk;
+    uint32_t err = 0;
+
+    for (;;) {
+        const uint32_t item = batch_next_item(ba, &s_item);
+        if (item >= ba.nitems)
+            break;
+        const BatchItem it = items[item];
+        const HaloCells h = halo_cells(it.icell, cell_offsets, ba.nycells, ba.nxcells, ba.nx, Periodic);
+        if (h.own < 0 && !ba.partial)
+            err |= ERR_NOT_IN_PIXELIZATION;
+        stage_halo_tile<NT>(tile, lmap, h, ba.ystride, ba.polstride, opaque_tid());
+        __syncthreads();
+
+        const int ybase = int(it.icell >> 10) << 6, xbase = int(it.icell & 1023u) << 6;
+        const uint32_t wb = uint32_t((uint64_t(it.nent) * wave) / (NT / WAVE));
+        const uint32_t we = uint32_t((uint64_t(it.nent) * (wave + 1)) / (NT / WAVE));
+        batch_wave_segments(segs, it, wb, we, [&](const BatchSeg &sg, uint32_t b, uint32_t e) {
+            const BatchTod td = tods[sg.tod];
+            m2t_entries<Periodic, true, false, TT>(static_cast<TT *>(td.tod), td.xp, td.plan_mt, td.qmask, td.nsamp,
+                                                   b, e, tile, sct, g, ybase, xbase, ba.accum, batch_gains(td), chk,
+                                                   err);
+        });
+        __syncthreads();     // the tile and s_item are rewritten for the next item
+        if (!ba.persistent)
+            break;
+    }
+    batch_finish(ba);
+    report_err(err | chk.bits(), ba.err_word);
+}
+
+template<bool Periodic, typename TT>
+__global__ void __launch_bounds__(T2M_THREADS)
+batched_tod2map_kernel(float *__restrict__ lmap, const BatchTod *__restrict__ tods,
+                       const BatchSeg *__restrict__ segs, const BatchItem *__restrict__ items,
+                       const int64_t *__restrict__ cell_offsets, BatchArgs ba)
+{
+    __shared__ unsigned long long acc[3 * HPLANE];       // 99 KiB
+    __shared__ float2 sct[SCT];
+    __shared__ uint32_t s_max, s_redo[3], s_item;
+    float *facc = reinterpret_cast<float *>(acc);
+
+    fill_sincos_table(sct, T2M_THREADS);
+    const Geom g = make_geom(ba.ny, ba.nx, Periodic);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    uint32_t err = 0;
+
+    for (;;) {
+        if (threadIdx.x == 1)
+            s_max = 0;
+        else if (threadIdx.x >= 2 && threadIdx.x < 5)
+            s_redo[threadIdx.x - 2] = 0;
+        if (!ba.persistent)
+            __syncthreads();
+        const uint32_t item = batch_next_item(ba, &s_item);
+        if (item >= ba.nitems)
+            break;
+        const BatchItem it = items[item];
+        const HaloCells h = halo_cells(it.icell, cell_offsets, ba.nycells, ba.nxcells, ba.nx, Periodic);
+        if (h.own < 0 && !ba.partial)
+            err |= ERR_NOT_IN_PIXELIZATION;
+        const int ybase = int(it.icell >> 10) << 6, xbase = int(it.icell & 1023u) << 6;
+        const uint32_t wb = uint32_t((uint64_t(it.nent) * wave) / (T2M_THREADS / WAVE));
+        const uint32_t we = uint32_t((uint64_t(it.nent) * (wave + 1)) / (T2M_THREADS / WAVE));
+
+        // Probe: the first lines of the wave's first segment.
+        {
+            float m = 0.0f;
+            bool first = true;
+            batch_wave_segments(segs, it, wb, we, [&](const BatchSeg &sg, uint32_t b, uint32_t e) {
+                if (first) {
+                    const BatchTod td = tods[sg.tod];
+                    m = t2m_probe<true, false, TT>(static_cast<const TT *>(td.tod), td.plan_mt, td.nsamp, b, e,
+                                                   batch_gains(td));
+                    first = false;
+                }
+            });
+            const uint32_t mw = wave_max(__float_as_uint(m));
+            if (lane == 0)
+                atomicMax(&s_max, mw);
+        }
+        t2m_clear<T2M_THREADS>(acc, true, opaque_tid());
+        __syncthreads();
+
+        float inv_scale;
+        const bool finite = t2m_accumulate<T2M_THREADS>(
+            acc, &s_max, s_redo,
+            [&](bool fin, float scale, float &mrun, float &tsum, Checks &chk) {
+                batch_wave_segments(segs, it, wb, we, [&](const BatchSeg &sg, uint32_t b, uint32_t e) {
+                    const BatchTod td = tods[sg.tod];
+                    t2m_entries<Periodic, true, false, TT>(acc, facc, static_cast<const TT *>(td.tod), td.xp,
+                                                           td.plan_mt, td.qmask, td.nsamp, b, e, sct, g, ybase, xbase,
+                                                           batch_gains(td), fin, scale, mrun, tsum, chk, err);
+                });
+            },
+            inv_scale, err, opaque_tid());
+        t2m_flush<T2M_THREADS>(lmap, acc, facc, h, (it.flags & CHUNK_SHARED) != 0, finite, inv_scale, ba.ystride,
+                               ba.polstride, opaque_tid());
+        __syncthreads();     // the tile and the block-shared words are reset for the next item
+        if (!ba.persistent)
+            break;
+    }
+    batch_finish(ba);
+    report_err(err, ba.err_word);
 }
 
 template<bool HasResp, typename TT>
@@ -903,6 +1187,54 @@ void tod2map_typed(float *lmap, const TT *tod, const float *xp, const OpArgs &a,
         launch_t2m<false, TT>(lmap, tod, xp, a, plan, gn, stream);
 }
 
+BatchArgs batch_args(const BatchLaunch &bl, const gmm_lpix &lp, bool partial, bool accum)
+{
+    BatchArgs ba;
+    ba.nitems = uint32_t(bl.nitems);
+    ba.persistent = (bl.grid > 0) ? 1u : 0u;
+    ba.counters = bl.counters;
+    ba.err_word = bl.err_word;
+    ba.ny = int(lp.nypix_global);
+    ba.nx = int(lp.nxpix_global);
+    ba.nycells = int(lp.nycells);
+    ba.nxcells = int(lp.nxcells);
+    ba.ystride = lp.ystride;
+    ba.polstride = lp.polstride;
+    ba.partial = partial;
+    ba.accum = accum;
+    return ba;
+}
+
+unsigned batch_grid(const BatchLaunch &bl)
+{
+    return unsigned((bl.grid > 0) ? std::min<int64_t>(bl.grid, bl.nitems) : bl.nitems);
+}
+
+template<typename TT>
+void batched_m2t_typed(const BatchLaunch &bl, const float *lmap, const int64_t *cell_offsets, const BatchArgs &ba,
+                       bool periodic, hipStream_t stream)
+{
+    const unsigned grid = batch_grid(bl);
+    if (bl.threads == 256) {
+        auto k = periodic ? batched_map2tod_kernel<true, TT, 256> : batched_map2tod_kernel<false, TT, 256>;
+        k<<<grid, 256, 0, stream>>>(lmap, bl.tods, bl.segs, bl.items, cell_offsets, ba);
+    }
+    else {
+        auto k = periodic ? batched_map2tod_kernel<true, TT, 512> : batched_map2tod_kernel<false, TT, 512>;
+        k<<<grid, 512, 0, stream>>>(lmap, bl.tods, bl.segs, bl.items, cell_offsets, ba);
+    }
+    GMM_HIP(hipGetLastError());
+}
+
+template<typename TT>
+void batched_t2m_typed(const BatchLaunch &bl, float *lmap, const int64_t *cell_offsets, const BatchArgs &ba,
+                       bool periodic, hipStream_t stream)
+{
+    auto k = periodic ? batched_tod2map_kernel<true, TT> : batched_tod2map_kernel<false, TT>;
+    k<<<batch_grid(bl), T2M_THREADS, 0, stream>>>(lmap, bl.tods, bl.segs, bl.items, cell_offsets, ba);
+    GMM_HIP(hipGetLastError());
+}
+
 }  // namespace
 
 // ==========================================================================================
@@ -926,6 +1258,35 @@ void launch_planned_tod2map(float *lmap, const void *tod, const float *xp, const
     case TOD_F16: tod2map_typed(lmap, static_cast<const _Float16 *>(tod), xp, a, plan, g, stream); break;
     case TOD_BF16: tod2map_typed(lmap, static_cast<const __bf16 *>(tod), xp, a, plan, g, stream); break;
     default: fail("tod2map: unknown TOD element type");
+    }
+}
+
+void launch_batched_map2tod(const BatchLaunch &bl, const float *lmap, const gmm_lpix &lp, int tod_type, bool partial,
+                            bool accum, hipStream_t stream)
+{
+    if (bl.nitems <= 0)
+        return;
+    GMM_REQUIRE(bl.threads == 256 || bl.threads == 512, "batched map2tod: threads per block must be 256 or 512");
+    const BatchArgs ba = batch_args(bl, lp, partial, accum);
+    switch (tod_type) {
+    case TOD_F32: batched_m2t_typed<float>(bl, lmap, lp.offsets_gpu, ba, lp.periodic_xcoord, stream); break;
+    case TOD_F16: batched_m2t_typed<_Float16>(bl, lmap, lp.offsets_gpu, ba, lp.periodic_xcoord, stream); break;
+    case TOD_BF16: batched_m2t_typed<__bf16>(bl, lmap, lp.offsets_gpu, ba, lp.periodic_xcoord, stream); break;
+    default: fail("batched map2tod: unknown TOD element type");
+    }
+}
+
+void launch_batched_tod2map(const BatchLaunch &bl, float *lmap, const gmm_lpix &lp, int tod_type, bool partial,
+                            hipStream_t stream)
+{
+    if (bl.nitems <= 0)
+        return;
+    const BatchArgs ba = batch_args(bl, lp, partial, false);
+    switch (tod_type) {
+    case TOD_F32: batched_t2m_typed<float>(bl, lmap, lp.offsets_gpu, ba, lp.periodic_xcoord, stream); break;
+    case TOD_F16: batched_t2m_typed<_Float16>(bl, lmap, lp.offsets_gpu, ba, lp.periodic_xcoord, stream); break;
+    case TOD_BF16: batched_t2m_typed<__bf16>(bl, lmap, lp.offsets_gpu, ba, lp.periodic_xcoord, stream); break;
+    default: fail("batched tod2map: unknown TOD element type");
     }
 }
 

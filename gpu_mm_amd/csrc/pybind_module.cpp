@@ -9,7 +9,9 @@
 
 #include "gpu_mm_amd.h"
 
+#include <algorithm>
 #include <cstdint>
+#include <memory>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -396,6 +398,99 @@ void planned_tod2map_q16(py::object local_map, py::object tod, py::object tod_sc
                                   as_stream(stream)));
 }
 
+// ------------------------------------------------------------------------------------------
+// Batched operators.
+
+struct Batch
+{
+    gmm_batch *h = nullptr;
+    py::list refs;            // the pixelization, plans and arrays the batch points into
+    int64_t map_size = 0;
+
+    Batch(const LocalPixelization &lpix, py::object lpix_obj, py::list plans, py::list xpointings, py::list tods,
+          py::object scales, py::object config)
+    {
+        const char *where = "PlanBatch constructor";
+        const size_t n = py::len(plans);
+        if (n == 0 || py::len(xpointings) != n || py::len(tods) != n)
+            raise(std::string(where) + ": plans, xpointings and tods must be non-empty lists of equal length");
+        const bool q16 = !scales.is_none();
+        if (q16 && py::len(scales) != n)
+            raise(std::string(where) + ": scales must be None or a list with one entry per TOD");
+        std::vector<const gmm_plan *> ph;
+        std::vector<const float *> xph, sch;
+        std::vector<void *> th;
+        std::vector<int64_t> ns, nrows;
+        int tod_type = GMM_TOD_F32;
+        refs.append(lpix_obj);
+        for (size_t i = 0; i < n; i++) {
+            const PointingPlan &pl = plans[i].cast<const PointingPlan &>();
+            Arr x = xp_arr(xpointings[i], where, true);
+            int64_t tn;
+            if (q16) {
+                Tod16 t = tod16_args(tods[i], py::list(scales)[i], where);
+                if (i > 0 && t.type != tod_type)
+                    raise(std::string(where) + ": all TODs of a batch must have the same dtype");
+                tod_type = t.type;
+                tn = t.t.size;
+                th.push_back(t.t.ptr);
+                sch.push_back(static_cast<const float *>(t.sc.ptr));
+                nrows.push_back(t.sc.size);
+            }
+            else {
+                Arr t = tod_arr(tods[i], where, true);
+                tn = t.size;
+                th.push_back(t.ptr);
+            }
+            if (tn != x.size / 3)
+                raise(std::string(where) + ": TOD " + std::to_string(i) + " has " + std::to_string(tn)
+                      + " samples but its xpointing has " + std::to_string(x.size / 3));
+            ph.push_back(pl.h);
+            xph.push_back(static_cast<const float *>(x.ptr));
+            ns.push_back(tn);
+            refs.append(plans[i]);
+            refs.append(xpointings[i]);
+            refs.append(tods[i]);
+            if (q16)
+                refs.append(py::list(scales)[i]);
+        }
+        gmm_batch_config c;
+        gmm_batch_default_config(&c);
+        if (!config.is_none())
+            c = config.cast<gmm_batch_config>();
+        {
+            py::gil_scoped_release nogil;
+            check(gmm_batch_create(&h, lpix.h, int32_t(n), ph.data(), xph.data(), th.data(), ns.data(), tod_type,
+                                   q16 ? sch.data() : nullptr, q16 ? nrows.data() : nullptr, &c));
+        }
+        map_size = 3 * lpix.info().npix;
+    }
+    ~Batch() { gmm_batch_destroy(h); }
+
+    gmm_batch_info info() const
+    {
+        gmm_batch_info i;
+        check(gmm_batch_get_info(h, &i));
+        return i;
+    }
+
+    Arr map(py::handle m, const char *where) const
+    {
+        Arr a = map_arr(m, where, true);
+        if (a.size != map_size)
+            raise(std::string(where) + ": expected local map of size 3*npix=" + std::to_string(map_size) + ", got "
+                  + std::to_string(a.size));
+        return a;
+    }
+};
+
+struct Graph
+{
+    gmm_graph *h = nullptr;
+    py::object batch_ref, map_ref;
+    ~Graph() { gmm_graph_destroy(h); }
+};
+
 void tod16_encode(py::object out, py::object scale, py::object tod, uintptr_t stream)
 {
     const char *where = "tod16_encode";
@@ -656,6 +751,109 @@ PYBIND11_MODULE(_gpu_mm_amd_py, m)
         check(gmm_kernel_timer_read(which, &ms, &n));
         return py::make_tuple(ms, n);
     }, "Returns (total_ms, launches) of GMM_TIMED_MAP2TOD (0) or GMM_TIMED_TOD2MAP (1) since the last enable/read");
+    py::class_<gmm_batch_config>(m, "BatchConfig")
+        .def(py::init([]() { gmm_batch_config c; gmm_batch_default_config(&c); return c; }))
+        .def_readwrite("m2t_item_entries", &gmm_batch_config::m2t_item_entries)
+        .def_readwrite("m2t_threads", &gmm_batch_config::m2t_threads)
+        .def_readwrite("m2t_blocks_per_cu", &gmm_batch_config::m2t_blocks_per_cu)
+        .def_readwrite("t2m_item_entries", &gmm_batch_config::t2m_item_entries)
+        .def_readwrite("t2m_blocks_per_cu", &gmm_batch_config::t2m_blocks_per_cu)
+        .def("__repr__", [](const gmm_batch_config &c) {
+            std::ostringstream ss;
+            ss << "BatchConfig(m2t_item_entries=" << c.m2t_item_entries << ", m2t_threads=" << c.m2t_threads
+               << ", m2t_blocks_per_cu=" << c.m2t_blocks_per_cu << ", t2m_item_entries=" << c.t2m_item_entries
+               << ", t2m_blocks_per_cu=" << c.t2m_blocks_per_cu << ")";
+            return ss.str();
+        });
+
+    py::class_<Batch>(m, "Batch")
+        .def(py::init([](py::object lpix, py::list plans, py::list xpointings, py::list tods, py::object scales,
+                         py::object config) {
+                 return new Batch(lpix.cast<const LocalPixelization &>(), lpix, plans, xpointings, tods, scales,
+                                  config);
+             }),
+             py::arg("local_pixelization"), py::arg("plans"), py::arg("xpointings"), py::arg("tods"),
+             py::arg("scales") = py::none(), py::arg("config") = py::none())
+        .def_property_readonly("ntod", [](const Batch &b) { return b.info().ntod; })
+        .def_property_readonly("config", [](const Batch &b) { return b.info().config; })
+        .def_property_readonly("nentries", [](const Batch &b) { return b.info().nentries; })
+        .def_property_readonly("ncells", [](const Batch &b) { return b.info().ncells; })
+        .def_property_readonly("generation", [](const Batch &b) { return b.info().generation; })
+        .def("work_items", [](const Batch &b) {
+            const gmm_batch_info i = b.info();
+            py::dict d;
+            d["m2t_nitems"] = i.m2t_nitems;
+            d["m2t_nsegs"] = i.m2t_nsegs;
+            d["m2t_grid"] = i.m2t_grid;
+            d["t2m_nitems"] = i.t2m_nitems;
+            d["t2m_nsegs"] = i.t2m_nsegs;
+            d["t2m_grid"] = i.t2m_grid;
+            return d;
+        })
+        .def("configure", [](Batch &b, const gmm_batch_config &c) {
+            py::gil_scoped_release nogil;
+            check(gmm_batch_configure(b.h, &c));
+        })
+        .def("map2tod", [](Batch &b, py::object local_map, bool partial, bool accum, bool check_errors,
+                           uintptr_t stream) {
+            Arr m = b.map(local_map, "PlanBatch.map2tod");
+            py::gil_scoped_release nogil;
+            check(gmm_batch_map2tod(b.h, static_cast<const float *>(m.ptr), m.size, partial, accum, check_errors,
+                                    as_stream(stream)));
+        }, py::arg("local_map"), py::arg("partial_pixelization") = false, py::arg("accum") = false,
+           py::arg("check_errors") = true, py::arg("stream") = 0)
+        .def("tod2map", [](Batch &b, py::object local_map, bool partial, bool check_errors, uintptr_t stream) {
+            Arr m = b.map(local_map, "PlanBatch.tod2map");
+            py::gil_scoped_release nogil;
+            check(gmm_batch_tod2map(b.h, static_cast<float *>(m.ptr), m.size, partial, check_errors,
+                                    as_stream(stream)));
+        }, py::arg("local_map"), py::arg("partial_pixelization") = false, py::arg("check_errors") = true,
+           py::arg("stream") = 0)
+        .def("check_errflags", [](Batch &b, const std::string &where) {
+            check(gmm_batch_check_errflags(b.h, where.c_str()));
+        }, py::arg("where") = "PlanBatch")
+        .def("autotune", [](Batch &b, py::object map_in, py::object map_scratch, int reps, uintptr_t stream) {
+            Arr mi = b.map(map_in, "PlanBatch.autotune"), ms = b.map(map_scratch, "PlanBatch.autotune");
+            std::vector<gmm_batch_tune_entry> tab(256);
+            int32_t n = 0;
+            {
+                py::gil_scoped_release nogil;
+                check(gmm_batch_autotune(b.h, static_cast<const float *>(mi.ptr), static_cast<float *>(ms.ptr),
+                                         mi.size, reps, tab.data(), int32_t(tab.size()), &n, as_stream(stream)));
+            }
+            py::list out;
+            for (int32_t k = 0; k < std::min<int32_t>(n, int32_t(tab.size())); k++)
+                out.append(py::make_tuple(tab[size_t(k)].op == GMM_OP_MAP2TOD ? "map2tod" : "tod2map",
+                                          tab[size_t(k)].item_entries, tab[size_t(k)].threads,
+                                          tab[size_t(k)].blocks_per_cu, tab[size_t(k)].ms));
+            return out;
+        }, py::arg("map_in"), py::arg("map_scratch"), py::arg("reps") = 3, py::arg("stream") = 0)
+        .def("capture", [](py::object self, const std::string &op, py::object local_map, bool partial, bool accum) {
+            Batch &b = self.cast<Batch &>();
+            int code;
+            if (op == "map2tod")
+                code = GMM_OP_MAP2TOD;
+            else if (op == "tod2map")
+                code = GMM_OP_TOD2MAP;
+            else
+                raise("PlanBatch.capture: op must be 'map2tod' or 'tod2map'");
+            Arr m = b.map(local_map, "PlanBatch.capture");
+            auto g = std::make_unique<Graph>();
+            {
+                py::gil_scoped_release nogil;
+                check(gmm_batch_graph_create(&g->h, b.h, code, static_cast<float *>(m.ptr), m.size, partial, accum));
+            }
+            g->batch_ref = self;
+            g->map_ref = local_map;
+            return g;
+        }, py::arg("op"), py::arg("local_map"), py::arg("partial_pixelization") = false, py::arg("accum") = false);
+
+    py::class_<Graph>(m, "BatchGraph")
+        .def("launch", [](Graph &g, uintptr_t stream) {
+            py::gil_scoped_release nogil;
+            check(gmm_graph_launch(g.h, as_stream(stream)));
+        }, py::arg("stream") = 0);
+
     m.def("tod16_encode", &tod16_encode, py::arg("out"), py::arg("scale"), py::arg("tod"), py::arg("stream") = 0);
     m.def("unplanned_map2tod", &unplanned_map2tod, py::arg("tod"), py::arg("local_map"), py::arg("xpointing"),
           py::arg("local_pixelization"), py::arg("errflags"), py::arg("partial_pixelization"), py::arg("accum"),

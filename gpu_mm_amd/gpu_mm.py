@@ -573,6 +573,111 @@ class PointingPlan(_ext.PointingPlan):
 ####################################################################################################
 
 
+class PlanBatch:
+    """
+    Several TODs, each with its own PointingPlan and xpointing, bound to one local map, so that
+    one kernel launch runs map2tod (or tod2map) for all of them. Not in the reference, which
+    launches one TOD at a time.
+
+      PlanBatch(local_pixelization, plans, xpointings, tods, scales=None, config=None)
+
+      - plans, xpointings, tods: lists of equal length; TOD i has the samples of xpointing i and
+        plan i. TODs are float32 GPU tensors, or (with 'scales') float16 / bfloat16 ones, all of
+        one dtype, each with a float32 scale per row (see encode_tod16).
+      - config: BatchConfig (work-item size, block size and persistent grid per operator), or
+        None for the defaults; autotune() picks one by measurement.
+
+    Work items are runs of about item_entries plan entries of one map cell, gathered across the
+    TODs: the cell's halo tile is staged (map2tod) or accumulated and flushed (tod2map) once for
+    all TODs that touch it. map2tod writes every bound TOD (bit for bit what the plain per-TOD
+    map2tod writes); tod2map adds all of them into the map (equal to the per-TOD calls up to the
+    order of the float additions). The batch holds references to everything it points into.
+    Calls on one batch must not overlap in time.
+    """
+
+    def __init__(self, local_pixelization, plans, xpointings, tods, scales=None, config=None):
+        if not isinstance(local_pixelization, LocalPixelization):
+            raise RuntimeError(f"PlanBatch: expected LocalPixelization, got {local_pixelization!r}")
+        for p in plans:
+            if not isinstance(p, PointingPlan):
+                raise RuntimeError(f"PlanBatch: expected a list of PointingPlans, got {p!r}")
+        self.pixelization = local_pixelization
+        self.tods = list(tods)
+        self.scales = None if scales is None else list(scales)
+        self._b = _ext.Batch(local_pixelization, list(plans), list(xpointings), self.tods, self.scales, config)
+
+    @property
+    def ntod(self):
+        return self._b.ntod
+
+    @property
+    def config(self):
+        return self._b.config
+
+    @property
+    def nentries(self):
+        return self._b.nentries
+
+    def work_items(self):
+        """Work-item counts, segment counts and grid sizes of both operators."""
+        return self._b.work_items()
+
+    def configure(self, config):
+        self._b.configure(config)
+
+    def _arr(self, local_map):
+        if isinstance(local_map, LocalMap):
+            if not (local_map.pixelization == self.pixelization):
+                raise RuntimeError("PlanBatch: local_map has a different LocalPixelization")
+            return local_map.arr
+        return local_map
+
+    def map2tod(self, local_map, partial_pixelization=False, accum=False, debug=False):
+        """Every bound TOD (=, or += if accum) A_i * local_map."""
+        self._b.map2tod(self._arr(local_map), partial_pixelization, accum,
+                        check_errors=(_errflag_check or debug), stream=_stream())
+
+    def tod2map(self, local_map, partial_pixelization=False, debug=False):
+        """local_map += sum_i A_i^T * tod_i."""
+        self._b.tod2map(self._arr(local_map), partial_pixelization,
+                        check_errors=(_errflag_check or debug), stream=_stream())
+
+    def check_errflags(self, where="PlanBatch"):
+        self._b.check_errflags(where)
+
+    def autotune(self, local_map, reps=3):
+        """
+        Times every candidate configuration of both operators on this batch (work-item size
+        1024..16384 entries; map2tod blocks of 256 or 512 threads; one block per item or a
+        persistent grid of 1..4 blocks per CU) and keeps the fastest. map2tod runs on
+        'local_map' and overwrites the bound TODs with map2tod(local_map); tod2map runs on a
+        scratch map. Returns the measurements: (op, item_entries, threads, blocks_per_cu, ms).
+        """
+        arr = self._arr(local_map)
+        scratch = torch.zeros_like(arr)
+        return self._b.autotune(arr, scratch, reps, stream=_stream())
+
+    def capture(self, op, local_map, partial_pixelization=False, accum=False):
+        """
+        HIP graph of one batched launch ('map2tod' or 'tod2map') with these arguments; replay it
+        with .launch() (on torch's current stream). The graph keeps the batch and the map alive,
+        and is refused after configure() or autotune().
+        """
+        g = self._b.capture(op, self._arr(local_map), partial_pixelization, accum)
+        return _BatchGraph(g)
+
+
+class _BatchGraph:
+    def __init__(self, g):
+        self._g = g
+
+    def launch(self):
+        self._g.launch(stream=_stream())
+
+
+BatchConfig = _ext.BatchConfig
+
+
 def read_xpointing_npzfile(filename):
     """
     Reads an xpointing array from the reference's Nov-2023 .npz format (key 'xpointing', rows

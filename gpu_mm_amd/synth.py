@@ -114,3 +114,83 @@ def raster_xpointing_torch(ndet, nt, *, x0, xamp, xperiod, y0, yspeed, radius, a
         xp[1, d0:d1] = (xs[None, :] + dxt[d0:d1, None]).to(torch.float32)
         xp[2, d0:d1] = (al[None, :] + tht[d0:d1, None]).to(torch.float32)
     return xp, rng
+
+
+# ---------------------------------------------------------------------------------------------
+# Batched uneven workload: many TODs of different sizes, each with its own rotated raster scan,
+# on one shared map.
+#
+# TOD i: a scan frame (u, v) with u = amp * tri(t / 8192) (the sweep) and v = 0.0025 t (the
+# drift), rotated by phi: y = y0 + u sin(phi) + v cos(phi) + dy_det, x = x0 + u cos(phi) -
+# v sin(phi) + dx_det; detector offsets uniform in a disk of radius 20 px and alpha = theta_det
+# + 0.001 t (the offset/alpha model of the 64 x 65536 raster). (y0, x0) is drawn uniformly among
+# the positions that keep the whole footprint (plus the offset radius and a 2-pixel margin)
+# inside 0 <= y <= ny-1, 0 <= x <= nx-2; phi is redrawn (up to 64 times) while the footprint
+# does not fit, and then the amplitude is halved until it does.
+
+BATCH_SCAN = dict(xperiod=8192.0, speed=0.0025, radius=20.0, alpha_rate=0.001, margin=2.0)
+
+CFG2 = dict(ny=2048, nx=4096, ntod=24, ndet_choices=tuple(range(96, 1537, 32)),
+            nt_choices=tuple(range(16384, 262145, 64)), amp_range=(300.0, 1500.0))
+
+
+def batch_scan_params(rng, ndet, nt, ny, nx, amp_range):
+    """Draws one TOD's scan: returns dict(y0, x0, amp, phi) (float64)."""
+    s = BATCH_SCAN
+    amp = rng.uniform(*amp_range)
+    vmax = s["speed"] * (nt - 1)
+    pad = s["radius"] + s["margin"]
+    for attempt in range(10000):
+        phi = rng.uniform(0.0, np.pi)
+        sn, cs = np.sin(phi), np.cos(phi)
+        uu = np.array([-amp, amp, -amp, amp])
+        vv = np.array([0.0, 0.0, vmax, vmax])
+        ys = uu * sn + vv * cs
+        xs = uu * cs - vv * sn
+        ylo, yhi = pad - ys.min(), (ny - 1) - pad - ys.max()
+        xlo, xhi = pad - xs.min(), (nx - 2) - pad - xs.max()
+        if ylo <= yhi and xlo <= xhi:
+            return dict(y0=rng.uniform(ylo, yhi), x0=rng.uniform(xlo, xhi), amp=amp, phi=phi)
+        if attempt % 64 == 63:
+            amp *= 0.5
+    raise RuntimeError("batch_scan_params: the map is too small for this scan")
+
+
+def batch_xpointing_torch(ndet, nt, scan, seed, device):
+    """xpointing float32 (3, ndet, nt) on 'device' for one TOD of the batched workload
+    (detector offsets and angles from np.random.default_rng(seed))."""
+    import torch
+    s = BATCH_SCAN
+    rng = np.random.default_rng(seed)
+    dy, dx = disk_offsets(rng, ndet, s["radius"])
+    theta = rng.uniform(0.0, np.pi, ndet)
+    t = torch.arange(nt, dtype=torch.float64, device=device)
+    w = t / s["xperiod"]
+    u = scan["amp"] * (1.0 - 4.0 * torch.abs((w - torch.floor(w)) - 0.5))
+    v = s["speed"] * t
+    sn, cs = float(np.sin(scan["phi"])), float(np.cos(scan["phi"]))
+    ys = scan["y0"] + u * sn + v * cs
+    xs = scan["x0"] + u * cs - v * sn
+    al = s["alpha_rate"] * t
+    xp = torch.empty((3, ndet, nt), dtype=torch.float32, device=device)
+    dyt = torch.from_numpy(dy).to(device)
+    dxt = torch.from_numpy(dx).to(device)
+    tht = torch.from_numpy(theta).to(device)
+    step = max(1, (1 << 26) // nt)
+    for d0 in range(0, ndet, step):
+        d1 = min(ndet, d0 + step)
+        xp[0, d0:d1] = (ys[None, :] + dyt[d0:d1, None]).to(torch.float32)
+        xp[1, d0:d1] = (xs[None, :] + dxt[d0:d1, None]).to(torch.float32)
+        xp[2, d0:d1] = (al[None, :] + tht[d0:d1, None]).to(torch.float32)
+    return xp
+
+
+def batch_shapes(seed, ntod, ndet_choices, nt_choices, ny, nx, amp_range):
+    """Per-TOD (ndet, nt, scan) of a batched workload, drawn from np.random.default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(ntod):
+        ndet = int(rng.choice(ndet_choices))
+        nt = int(rng.choice(nt_choices))
+        out.append((ndet, nt, batch_scan_params(rng, ndet, nt, ny, nx, amp_range)))
+    return out

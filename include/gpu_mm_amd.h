@@ -254,6 +254,88 @@ int gmm_planned_tod2map_q16(float *local_map_gpu, int64_t local_map_size,
 int gmm_tod16_encode(void *out_gpu, int tod_type, float *scale_gpu, const float *tod_gpu, int64_t nrows,
                      int64_t row_len, void *stream);
 
+/* ---- Batched operators -------------------------------------------------------------------
+ * No counterpart in the reference, which launches one TOD at a time (map2tod.cu:198-272,
+ * tod2map.cu:160-226). A batch binds several TODs, each with its own plan and xpointing, to one
+ * LocalPixelization; one kernel launch then runs map2tod (or tod2map) for all of them. The work
+ * items are runs of about item_entries plan entries of one cell gathered across the TODs, so
+ * the cell's halo tile is staged (map2tod) or accumulated and flushed (tod2map) once for all the
+ * TODs that touch it. With blocks_per_cu > 0 the launch is a persistent grid of that many blocks
+ * per CU taking items from a device queue, largest first; with 0 it has one block per item.
+ * Results equal those of the plain per-TOD calls (map2tod bit for bit; tod2map up to the
+ * order of the float additions into the map). The plans, xpointing, TOD and scale arrays must
+ * outlive the batch; TODs and xpointing must be 16-byte aligned (8 bytes for 16-bit TODs).
+ * Calls on one batch must not overlap in time (they share its queue counters). */
+typedef struct gmm_batch gmm_batch;
+typedef struct gmm_graph gmm_graph;
+
+enum { GMM_TOD_F32 = 0 };
+enum { GMM_OP_MAP2TOD = 0, GMM_OP_TOD2MAP = 1 };
+
+typedef struct {
+    int32_t m2t_item_entries;    /* plan entries per map2tod work item (64 .. 2^24) */
+    int32_t m2t_threads;         /* map2tod threads per block: 256 or 512 */
+    int32_t m2t_blocks_per_cu;   /* map2tod persistent grid, blocks per CU (0 = one block per item) */
+    int32_t t2m_item_entries;    /* plan entries per tod2map work item */
+    int32_t t2m_blocks_per_cu;   /* tod2map persistent grid, blocks per CU (0 = one block per item) */
+    int32_t reserved[3];
+} gmm_batch_config;
+
+typedef struct {
+    int32_t ntod, tod_type;
+    gmm_batch_config config;
+    int64_t nentries;            /* plan entries over all TODs */
+    int64_t ncells;              /* distinct cells */
+    int64_t m2t_nitems, m2t_nsegs, m2t_grid;
+    int64_t t2m_nitems, t2m_nsegs, t2m_grid;
+    int64_t generation;          /* incremented by every gmm_batch_configure() */
+} gmm_batch_info;
+
+typedef struct {
+    int32_t op;                  /* GMM_OP_MAP2TOD or GMM_OP_TOD2MAP */
+    int32_t item_entries, threads, blocks_per_cu;
+    double ms;                   /* median kernel time */
+} gmm_batch_tune_entry;
+
+void gmm_batch_default_config(gmm_batch_config *cfg);
+
+/* tod_type: GMM_TOD_F32, GMM_TOD_F16 or GMM_TOD_BF16 (then tod_scale_gpu[i] is float[tod_nrows[i]],
+ * otherwise both may be null). nsamp[i] is the sample count of TOD i (its plan's nsamp). */
+int gmm_batch_create(gmm_batch **out, const gmm_lpix *lpix, int32_t ntod, const gmm_plan *const *plans,
+                     const float *const *xpointing_gpu, void *const *tod_gpu, const int64_t *nsamp,
+                     int32_t tod_type, const float *const *tod_scale_gpu, const int64_t *tod_nrows,
+                     const gmm_batch_config *config);
+void gmm_batch_destroy(gmm_batch *b);
+/* Rebuilds the work items for a new configuration (invalidates graphs captured before). */
+int gmm_batch_configure(gmm_batch *b, const gmm_batch_config *config);
+int gmm_batch_get_info(const gmm_batch *b, gmm_batch_info *out);
+
+/* map2tod into every bound TOD (overwrite, or add with accum) / tod2map from every bound TOD
+ * into local_map (always accumulates). check_errors: synchronize and raise on error flags. */
+int gmm_batch_map2tod(gmm_batch *b, const float *local_map_gpu, int64_t local_map_size, int partial_pixelization,
+                      int accum, int check_errors, void *stream);
+int gmm_batch_tod2map(gmm_batch *b, float *local_map_gpu, int64_t local_map_size, int partial_pixelization,
+                      int check_errors, void *stream);
+/* Synchronous check (and reset) of the error flags accumulated by the batch's launches. */
+int gmm_batch_check_errflags(gmm_batch *b, const char *where);
+
+/* Autotuner: times each candidate configuration of each operator (reps launches, median; one
+ * untimed launch first) and configures the batch with the fastest. map2tod is timed reading
+ * map_in_gpu and writing the bound TODs (which end up holding map2tod(map_in)); tod2map is timed
+ * accumulating into map_scratch_gpu. The table receives every measurement (up to capacity). */
+int gmm_batch_autotune(gmm_batch *b, const float *map_in_gpu, float *map_scratch_gpu, int64_t local_map_size,
+                       int32_t reps, gmm_batch_tune_entry *table, int32_t table_capacity, int32_t *table_size,
+                       void *stream);
+
+/* HIP graph of one batched operator launch (op = GMM_OP_MAP2TOD or GMM_OP_TOD2MAP) with fixed
+ * arguments, captured on a private stream. The persistent grid rewinds its own queue at the end
+ * of every launch, so the graph needs no reset node. The graph must not outlive the batch, and
+ * is refused after the batch is reconfigured. */
+int gmm_batch_graph_create(gmm_graph **out, gmm_batch *b, int32_t op, float *local_map_gpu, int64_t local_map_size,
+                           int partial_pixelization, int accum);
+int gmm_graph_launch(gmm_graph *g, void *stream);
+void gmm_graph_destroy(gmm_graph *g);
+
 /* launch_unplanned_map2tod, include/gpu_mm.hpp:204-213; src_lib/map2tod_unplanned.cu:74-127.
  * errflags_gpu: caller-supplied uint32[nerrflags]; the launch uses at most nerrflags blocks. */
 int gmm_unplanned_map2tod(float *tod_gpu, int64_t tod_nsamp,
