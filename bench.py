@@ -207,6 +207,193 @@ def cpu_baseline(w, ndet_sample):
                 multicore=dict(value=round(mt, 5), cores=nthr, threads_kind="OpenMP, tod2map with private maps"))
 
 
+def batched_leg(torch, gmm, synth, dev, steps, warmup, ntod, err_ndet):
+    """
+    Batched uneven workload: ntod TODs (ndet uniform over multiples of 32 in [96, 1536], nt over
+    multiples of 64 in [16384, 262144], each with its own rotated raster scan) on one dense
+    3x2048x4096 map (Gaussian random field), TOD values N(0,1). A step is map2tod into every TOD
+    plus tod2map from every TOD, run as (a) one plain planned launch per TOD and operator, (b)
+    one autotuned batched launch per operator, eagerly and replayed from HIP graphs, and (c) the
+    batched launches on FP16 and BF16 TODs with per-detector FP32 scales.
+    """
+    c = synth.CFG2
+    ny, nx = c["ny"], c["nx"]
+    shapes = synth.batch_shapes(2, ntod, c["ndet_choices"], c["nt_choices"], ny, nx, c["amp_range"])
+    t0 = time.time()
+    xps, plans, ins, outs = [], [], [], []
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(5)
+    for i, (ndet, nt, scan) in enumerate(shapes):
+        xg = synth.batch_xpointing_torch(ndet, nt, scan, seed=1000 + i, device=dev)
+        plans.append(gmm.PointingPlan(gmm.PointingPrePlan(xg, ny, nx, periodic_xcoord=False), xg))
+        xps.append(xg)
+        ins.append(torch.randn((ndet, nt), generator=gen, device=dev, dtype=torch.float32))
+        outs.append(torch.empty((ndet, nt), device=dev, dtype=torch.float32))
+    lp = gmm.LocalPixelization.make_rectangle(ny, nx, periodic_xcoord=False)
+    lmap = gmm.LocalMap(lp, synth.grf_map_torch(ny, nx, seed=6, device=dev).reshape(-1).contiguous())
+    acc = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
+    torch.cuda.synchronize()
+    nsamp = sum(a * b for a, b, _ in shapes)
+    tod_cells = 0
+    for p in plans:
+        mt = p.get_plan_mt() & np.uint64(0xFFFFF)
+        tod_cells += int(np.count_nonzero(np.diff(mt)) + (mt.size > 0))
+    log(f"[batch] {ntod} TODs, {nsamp / 1e9:.2f}e9 samples, plans built in {time.time() - t0:.1f}s")
+
+    def timed(step):
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            step()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    def op_times(step_m2t, step_t2m):
+        return timed(step_m2t), timed(step_t2m)
+
+    res = {}
+    # (a) plain: one launch per TOD and operator.
+    def plain_m2t():
+        for x, p, o in zip(xps, plans, outs):
+            gmm.map2tod(o, lmap, x, p)
+
+    def plain_t2m():
+        for x, p, t in zip(xps, plans, ins):
+            gmm.tod2map(acc, t, x, p)
+
+    ms = timed(lambda: (plain_m2t(), plain_t2m()))
+    m_ms, t_ms = op_times(plain_m2t, plain_t2m)
+    res["plain_f32"] = dict(ms_per_step=ms, map2tod_ms=m_ms, tod2map_ms=t_ms, cells=tod_cells)
+
+    # (b) batched FP32, autotuned.
+    bm = gmm.PlanBatch(lp, plans, xps, outs)
+    bt = gmm.PlanBatch(lp, plans, xps, ins)
+    t0 = time.time()
+    table = bm.autotune(lmap, reps=3)
+    bt.configure(bm.config)
+    log(f"[batch] autotune ({len(table)} configurations) in {time.time() - t0:.1f}s: {bm.config}")
+    ms = timed(lambda: (bm.map2tod(lmap), bt.tod2map(acc)))
+    m_ms, t_ms = op_times(lambda: bm.map2tod(lmap), lambda: bt.tod2map(acc))
+    gm, gt = bm.capture("map2tod", lmap), bt.capture("tod2map", acc)
+    ms_graph = timed(lambda: (gm.launch(), gt.launch()))
+    cfg = bm.config
+    res["batched_f32"] = dict(ms_per_step=ms, graph_ms_per_step=ms_graph, map2tod_ms=m_ms, tod2map_ms=t_ms,
+                              cells=int(bm._b.ncells), work_items=bm.work_items(),
+                              config=dict(m2t_item_entries=cfg.m2t_item_entries, m2t_threads=cfg.m2t_threads,
+                                          m2t_blocks_per_cu=cfg.m2t_blocks_per_cu,
+                                          t2m_item_entries=cfg.t2m_item_entries,
+                                          t2m_blocks_per_cu=cfg.t2m_blocks_per_cu))
+    del gm, gt
+
+    # (c) batched 16-bit TODs (per-detector scales; output scales fitted to the FP32 map2tod).
+    bm.map2tod(lmap)
+    torch.cuda.synchronize()
+    cfg16 = None
+    for form, dt in (("f16", torch.float16), ("bf16", torch.bfloat16)):
+        enc_in = [gmm.encode_tod16(t, dt) for t in ins]
+        enc_out = [gmm.encode_tod16(o, dt) for o in outs]
+        b16m = gmm.PlanBatch(lp, plans, xps, [e[0] for e in enc_out], scales=[e[1] for e in enc_out])
+        b16t = gmm.PlanBatch(lp, plans, xps, [e[0] for e in enc_in], scales=[e[1] for e in enc_in])
+        if cfg16 is None:
+            b16m.autotune(lmap, reps=3)
+            cfg16 = b16m.config
+            log(f"[batch] autotune {form}: {cfg16}")
+        b16m.configure(cfg16)
+        b16t.configure(cfg16)
+        ms = timed(lambda: (b16m.map2tod(lmap), b16t.tod2map(acc)))
+        m_ms, t_ms = op_times(lambda: b16m.map2tod(lmap), lambda: b16t.tod2map(acc))
+        res[f"batched_{form}"] = dict(ms_per_step=ms, map2tod_ms=m_ms, tod2map_ms=t_ms)
+        del b16m, b16t, enc_in, enc_out
+        torch.cuda.empty_cache()
+
+    for k, v in res.items():
+        bps = 16 if k.endswith("f32") else 14
+        cells = res["plain_f32"]["cells"] if k == "plain_f32" else res["batched_f32"]["cells"]
+        v["gsamp_s"] = nsamp / (v["ms_per_step"] * 1e-3) / 1e9
+        for op, mult in (("map2tod", 1), ("tod2map", 2)):
+            byt = bps * nsamp + mult * CELL_BYTES * cells
+            v[f"{op}_frac_hbm"] = byt / (v[f"{op}_ms"] * 1e-3) / HBM_PEAK
+        log(f"[batch] {k}: {v['ms_per_step']:.3f} ms/step {v['gsamp_s']:.1f} Gsamp/s; map2tod {v['map2tod_ms']:.3f} ms "
+            f"({100 * v['map2tod_frac_hbm']:.1f}% HBM), tod2map {v['tod2map_ms']:.3f} ms "
+            f"({100 * v['tod2map_frac_hbm']:.1f}% HBM)" + (f"; graph {v['graph_ms_per_step']:.3f} ms/step"
+                                                          if "graph_ms_per_step" in v else ""))
+
+    errs = batched_errors(torch, gmm, lp, lmap, xps, ins, ny, nx, err_ndet) if err_ndet > 0 else {}
+    out = {
+        "workload": f"{ntod} TODs (ndet 96..1536, nt 16384..262144, rotated raster scans) on a dense 3x{ny}x{nx} map",
+        "ntod": ntod, "nsamp": nsamp, "steps": steps, "warmup": warmup,
+        "forms": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()} for k, v in res.items()},
+        "speedup_vs_plain_f32": {k: round(res["plain_f32"]["ms_per_step"] / v["ms_per_step"], 3) for k, v in res.items()},
+        "errors": errs,
+    }
+    del bm, bt, plans, xps, ins, outs
+    torch.cuda.empty_cache()
+    return out
+
+
+def batched_errors(torch, gmm, lp, lmap, xps, ins, ny, nx, ndet_err):
+    """Map (tod2map) and TOD (map2tod) errors of the batched forms against the FP64 oracle, on a
+    batch of the first ndet_err detectors of every TOD."""
+    from oracle import pyoracle as orc
+    xs = [x[:, :ndet_err].contiguous() for x in xps]
+    ts = [t[:ndet_err].contiguous() for t in ins]
+    plans = [gmm.PointingPlan(gmm.PointingPrePlan(x, ny, nx, periodic_xcoord=False), x) for x in xs]
+    pix = orc.Pix(ny, nx, lp.cell_offsets_cpu, lp.ystride, lp.polstride, False)
+    t0 = time.perf_counter()
+    m_ref = None
+    t_ref = []
+    mh = lmap.arr.cpu().numpy().astype(np.float64)
+    for x, t in zip(xs, ts):
+        xh = x.cpu().numpy()
+        m_ref, e1 = orc.tod2map(t.cpu().numpy().astype(np.float64), xh, pix, lmap=m_ref, f64=True)
+        tr, e2 = orc.map2tod(mh, xh, pix, f64=True)
+        assert e1 == 0 and e2 == 0
+        t_ref.append(tr)
+    t_ref = np.concatenate(t_ref)
+    log(f"[batch] FP64 oracle on {ndet_err} detectors of each TOD: {time.perf_counter() - t0:.1f}s")
+    npix = lp.npix
+    hit = m_ref[:npix] != 0.0
+    out = {}
+    for form in ("f32", "f16", "bf16"):
+        if form == "f32":
+            src, ssc = ts, None
+            dst = [torch.empty_like(t) for t in ts]
+            dsc = None
+        else:
+            dt = torch.float16 if form == "f16" else torch.bfloat16
+            enc = [gmm.encode_tod16(t, dt) for t in ts]
+            src, ssc = [e[0] for e in enc], [e[1] for e in enc]
+            o32 = [torch.empty_like(t) for t in ts]
+            gmm.PlanBatch(lp, plans, xs, o32).map2tod(lmap)
+            enc_o = [gmm.encode_tod16(o, dt) for o in o32]
+            dst, dsc = [e[0] for e in enc_o], [e[1] for e in enc_o]
+        m = gmm.LocalMap(lp, torch.zeros_like(lmap.arr))
+        gmm.PlanBatch(lp, plans, xs, src, scales=ssc).tod2map(m)
+        gmm.PlanBatch(lp, plans, xs, dst, scales=dsc).map2tod(lmap)
+        got_m = m.arr.cpu().numpy().astype(np.float64)
+        if dsc is None:
+            got_t = np.concatenate([d.cpu().numpy().astype(np.float64).ravel() for d in dst])
+        else:
+            got_t = np.concatenate([(d.float() * s[:, None]).cpu().numpy().astype(np.float64).ravel()
+                                    for d, s in zip(dst, dsc)])
+        rms_rel, max_rel = 0.0, 0.0
+        for cpt in range(3):
+            ref = m_ref[cpt * npix:(cpt + 1) * npix][hit]
+            d = got_m[cpt * npix:(cpt + 1) * npix][hit] - ref
+            r = float(np.sqrt(np.mean(ref ** 2)))
+            rms_rel = max(rms_rel, float(np.sqrt(np.mean(d ** 2))) / r)
+            max_rel = max(max_rel, float(np.max(np.abs(d))) / r)
+        tod_l1 = float(np.sum(np.abs(got_t - t_ref)) / np.sum(np.abs(t_ref)))
+        out[form] = dict(map_rms_rel=rms_rel, map_max_rel=max_rel, tod_l1_rel=tod_l1)
+        log(f"[batch] {form}: map error vs FP64 (RMS / max, relative to map RMS) {rms_rel:.2e} / {max_rel:.2e}; "
+            f"TOD L1-rel {tod_l1:.2e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +406,8 @@ def main():
     ap.add_argument("--cpu-ndet", type=int, default=1024,
                     help="detectors in the CPU baseline sample (about 12 s of single-thread work at 1024)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-ntod", type=int, default=24,
+                    help="TODs in the batched uneven workload (0: skip that leg)")
     ap.add_argument("--copy-ref", action="store_true",
                     help="also run a plain 1 GiB device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -279,6 +468,13 @@ def main():
 
     cpu = None if args.no_cpu else cpu_baseline(w, min(args.cpu_ndet, args.ndet))
 
+    batched = None
+    if args.batch_ntod > 0:
+        del w["tod"]
+        torch.cuda.empty_cache()
+        batched = batched_leg(torch, gmm, synth, dev, args.steps, args.warmup, args.batch_ntod,
+                              8 if args.err_ndet > 0 else 0)
+
     r = res[chosen]
     dom = max(r["ops"], key=lambda k: r["ops"][k]["seconds"])
     traffic = None
@@ -324,6 +520,7 @@ def main():
             "algorithmic_bytes_per_launch": int(r["ops"][dom]["bytes"]),
         },
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
 
