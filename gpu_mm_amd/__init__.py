@@ -1,6 +1,7 @@
 """gpu_mm_amd: planned pointing operators (map2tod / tod2map) for AMD MI355X.
 
-See gpu_mm_amd.gpu_mm for the data model and API.
+See gpu_mm_amd.gpu_mm for the data model and API, and gpu_mm_amd.mapmaker for the differentiable
+operators and the preconditioned CG map solver.
 """
 
 from .gpu_mm import (  # noqa: F401
@@ -11,5 +12,6 @@ from .gpu_mm import (  # noqa: F401
     PlanBatch, BatchConfig,
 )
 from . import gpu_mm  # noqa: F401
+from . import mapmaker  # noqa: F401
 
 __doc__ = gpu_mm.__doc__

@@ -194,3 +194,24 @@ def batch_shapes(seed, ntod, ndet_choices, nt_choices, ny, nx, amp_range):
         nt = int(rng.choice(nt_choices))
         out.append((ndet, nt, batch_scan_params(rng, ndet, nt, ny, nx, amp_range)))
     return out
+
+
+# ---------------------------------------------------------------------------------------------
+# Map-making workload: TODs whose scans cross at several angles, all centred on the map, so that
+# the crossings constrain Q and U. Scan model as for the batched workload (sweep amplitude 'amp',
+# drift 0.0025 px per sample), with the drift's midpoint at the map centre.
+
+CFG3 = dict(ny=1024, nx=2048, ndet=512, nt=131072, angles_deg=(0.0, 45.0, 90.0, 135.0), amp=300.0,
+            noise_var=(0.5, 2.0))
+
+
+def crossing_scans(ny, nx, nt, angles_deg, amp):
+    """Scan dicts (y0, x0, amp, phi) for batch_xpointing_torch, one per angle."""
+    s = BATCH_SCAN
+    vmax = s["speed"] * (nt - 1)
+    out = []
+    for a in angles_deg:
+        phi = np.deg2rad(a)
+        cy, cx = 0.5 * (ny - 1), 0.5 * (nx - 2)
+        out.append(dict(y0=cy - 0.5 * vmax * np.cos(phi), x0=cx + 0.5 * vmax * np.sin(phi), amp=amp, phi=phi))
+    return out
