@@ -76,6 +76,25 @@ __device__ inline int64_t chunk_cell_offset(const Chunk &ch, const int64_t *__re
     return (cy < nycells && cx < nxcells) ? cell_offsets[int64_t(cy) * nxcells + cx] : int64_t(-1);
 }
 
+// Work unit of this block. Workgroups are dispatched to the GPU's 8 XCDs round robin (block b on
+// XCD b mod 8), and each XCD has its own L2. Chunks are in cell order, and a line that crosses a
+// cell boundary is read by the blocks of both cells. Runs of XCD_RUN consecutive chunks are
+// mapped onto one XCD (blocks b, b + 8, ...), so that neighbouring cells are mostly processed
+// on one XCD at about the same time and the second read of such a line hits that XCD's L2,
+// while the runs still alternate between XCDs for balance. A bijection on [0, n): whole groups
+// of 8 * XCD_RUN blocks are permuted, the tail is left in place.
+constexpr uint32_t NUM_XCD = 8;
+constexpr uint32_t XCD_RUN = 4;      // measured: runs of 4 beat 1 (plain round robin), 16 and whole ranges
+
+__device__ inline uint32_t xcd_chunk(uint32_t b, uint32_t n)
+{
+    constexpr uint32_t G = NUM_XCD * XCD_RUN;
+    if (b >= (n / G) * G)
+        return b;
+    const uint32_t w = b % G;
+    return (b - w) + (w % NUM_XCD) * XCD_RUN + w / NUM_XCD;
+}
+
 // Slice [wb, we) of the chunk's entries handled by wave 'wave' of 'nw'.
 __device__ inline void wave_slice(const Chunk &ch, int wave, int nw, uint32_t &wb, uint32_t &we)
 {
@@ -228,34 +247,23 @@ __device__ inline void quad_scale(const Gains &g, uint32_t s0, float (&sc)[4])
 
 // ------------------------------------------------------------------------------------------
 // Sample location for the operators: the same index and offset arithmetic as locate_pixel()
-// (so the kernels agree with the planner bit for bit), with the range checks deferred. Per lane,
-//   chk_y = max(-y, y - (ny-1)) and chk_x = max(lo - x, x - hi) over its samples (> 0 iff some
-//   sample is out of range; the signs of these exact-sign subtractions are exact), and
-//   nan_y, nan_x = sums of the coordinates (NaN iff some coordinate is NaN, or infinite ones
-//   of both signs, which the range check flags anyway).
-
-struct Checks
-{
-    float y = -INFINITY, x = -INFINITY, ny = 0.0f, nx = 0.0f;
-
-    __device__ uint32_t bits() const
-    {
-        uint32_t e = 0;
-        if (!(y <= 0.0f) || !(ny == ny))
-            e |= ERR_BAD_YPIX;
-        if (!(x <= 0.0f) || !(nx == nx))
-            e |= ERR_BAD_XPIX;
-        return e;
-    }
-};
+// (so the kernels agree with the planner bit for bit), with the range checks deferred:
+//   - y (valid range [0, ny-1]) and non-periodic x ([0, nx-2]): each lane keeps the largest bit
+//     pattern of coordinate + 0.0f (the addition turns -0.0 into +0.0). Read as unsigned
+//     integers, the bit patterns of the floats in [0, hi] are exactly those <= bits(hi): any
+//     negative value has the sign bit set, and +Inf and every NaN lie above bits(hi).
+//   - periodic x ([-nx, 2nx]): the largest max(lo - x, x - hi) (> 0 iff out of range; these
+//     exact-sign subtractions have exact signs) and a sum of the coordinates (NaN iff some x is
+//     NaN).
 
 struct Geom
 {
     int ny, nx;
     float ymax;     // ny - 1
-    float xlo, xhi; // valid x range: [0, nx-2] or, periodic, [-nx, 2nx]
+    float xlo, xhi; // valid x range of the periodic case: [-nx, 2nx]
     float fnx;
     float fny2, fnx2; // ny - 2, nx - 2
+    uint32_t uymax, uxmax;  // bit patterns of ny - 1 and nx - 2
 };
 
 __device__ inline Geom make_geom(int ny, int nx, bool periodic)
@@ -269,33 +277,54 @@ __device__ inline Geom make_geom(int ny, int nx, bool periodic)
     g.fnx2 = float(nx - 2);
     g.xlo = periodic ? -float(nx) : 0.0f;
     g.xhi = periodic ? float(2 * nx) : float(nx - 2);
+    g.uymax = __float_as_uint(g.ymax);
+    g.uxmax = __float_as_uint(g.fnx2);
     return g;
+}
+
+struct Checks
+{
+    uint32_t uy = 0, ux = 0;
+    float px = -INFINITY, pn = 0.0f;
+
+    __device__ uint32_t bits(const Geom &g) const
+    {
+        uint32_t e = 0;
+        if (uy > g.uymax)
+            e |= ERR_BAD_YPIX;
+        if (ux > g.uxmax || !(px <= 0.0f) || !(pn == pn))
+            e |= ERR_BAD_XPIX;
+        return e;
+    }
+};
+
+// Clamp to [0, hi] in one instruction (v_med3_f32). For a NaN input the hardware returns the
+// minimum of the operands, which is 0 here or NaN; either converts to index 0.
+__device__ inline float clamp0(float v, float hi)
+{
+    return __builtin_amdgcn_fmed3f(v, 0.0f, hi);
 }
 
 template<bool Periodic>
 __device__ inline void locate_op(float y, float x, const Geom &g, Checks &c, int &iy0, int &ix0, float &dy, float &dx)
 {
-    c.y = fmaxf(c.y, fmaxf(-y, y - g.ymax));
-    c.x = fmaxf(c.x, fmaxf(g.xlo - x, x - g.xhi));
-    c.ny += y;
-    c.nx += x;
-
-    // Clamping in float before the conversion gives the same index as truncating and then
-    // clamping (NaN goes to 0 through fmaxf) for every in-range coordinate; the conversion is
-    // then always defined. Out-of-range coordinates are flagged above.
-    const int iy = int(fminf(fmaxf(y, 0.0f), g.fny2));
+    c.uy = max(c.uy, __float_as_uint(y + 0.0f));
+    const int iy = int(clamp0(y, g.fny2));
     dy = y - float(iy);
     iy0 = iy;
 
     if (Periodic) {
+        c.px = fmaxf(c.px, fmaxf(g.xlo - x, x - g.xhi));
+        c.pn += x;
         x = (x >= 0.0f) ? x : (x + g.fnx);
         x = (x <= g.fnx) ? x : (x - g.fnx);
-        const int ix = int(fminf(fmaxf(x, 0.0f), g.fnx));
+        const int ix = int(clamp0(x, g.fnx));
         dx = x - float(ix);
         ix0 = (ix < g.nx) ? ix : 0;
     }
     else {
-        const int ix = int(fminf(fmaxf(x, 0.0f), g.fnx2));
+        c.ux = max(c.ux, __float_as_uint(x + 0.0f));
+        const int ix = int(clamp0(x, g.fnx2));
         dx = x - float(ix);
         ix0 = ix;
     }
@@ -335,21 +364,29 @@ __device__ inline HaloCells halo_cells(uint32_t icell, const int64_t *__restrict
     return h;
 }
 
+// map2tod's tile holds each pixel's (T, Q, U) together in one float4 (the fourth lane unused),
+// so the four corners of a sample are four 16-byte LDS reads from one base address.
 template<int NT>
-__device__ inline void stage_halo_tile(float *__restrict__ tile, const float *__restrict__ lmap, const HaloCells &h,
+__device__ inline void stage_halo_tile(float4 *__restrict__ tile, const float *__restrict__ lmap, const HaloCells &h,
                                        int64_t ystride, int64_t polstride, int tid)
 {
     // Interior (column W excluded when W < 64: it holds the wrapped column).
-    for (int i = tid; i < 3 * CELL_NPIX; i += NT) {
-        const int p = i >> 12, r = (i >> 6) & 63, c = i & 63;
+    for (int i = tid; i < CELL_NPIX; i += NT) {
+        const int r = i >> 6, c = i & 63;
         if (c == h.w)
             continue;
-        tile[p * HPLANE + r * HS + c] = (h.own >= 0) ? lmap[h.own + p * polstride + r * ystride + c] : 0.0f;
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (h.own >= 0) {
+            const float *p = lmap + h.own + r * ystride + c;
+            v.x = p[0];
+            v.y = p[polstride];
+            v.z = p[2 * polstride];
+        }
+        tile[r * HS + c] = v;
     }
     // Row 64 (columns 0..W) and column W (rows 0..63).
     const int nrow = h.w + 1, nper = nrow + 64;
-    for (int i = tid; i < 3 * nper; i += NT) {
-        const int p = i / nper, j = i - p * nper;
+    for (int j = tid; j < nper; j += NT) {
         int64_t src;
         int idx;
         if (j < nrow) {
@@ -361,28 +398,33 @@ __device__ inline void stage_halo_tile(float *__restrict__ tile, const float *__
             idx = r * HS + h.w;
             src = (h.right >= 0) ? h.right + r * ystride : -1;
         }
-        tile[p * HPLANE + idx] = (src >= 0) ? lmap[src + p * polstride] : 0.0f;
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (src >= 0) {
+            const float *p = lmap + src;
+            v.x = p[0];
+            v.y = p[polstride];
+            v.z = p[2 * polstride];
+        }
+        tile[idx] = v;
     }
 }
 
-// T + cos2a Q + sin2a U (times the gains), bilinear over the four corners at tile offset 'o'.
+// T + cos2a Q + sin2a U (times the gains), bilinear over the four corners at tile index 'o'.
 template<bool HasResp>
-__device__ inline float m2t_interp(const float *__restrict__ tile, int o, float dy, float dx, float cs, float sn,
+__device__ inline float m2t_interp(const float4 *__restrict__ tile, int o, float dy, float dx, float cs, float sn,
                                    float tr, float pr)
 {
-    const float *tq = tile + HPLANE, *tu = tile + 2 * HPLANE;
     if (HasResp) {
         cs *= pr;
         sn *= pr;
     }
-    float t00 = tile[o], t01 = tile[o + 1], t10 = tile[o + HS], t11 = tile[o + HS + 1];
-    if (HasResp) {
-        t00 *= tr; t01 *= tr; t10 *= tr; t11 *= tr;
-    }
-    const float v00 = fmaf(sn, tu[o], fmaf(cs, tq[o], t00));
-    const float v01 = fmaf(sn, tu[o + 1], fmaf(cs, tq[o + 1], t01));
-    const float v10 = fmaf(sn, tu[o + HS], fmaf(cs, tq[o + HS], t10));
-    const float v11 = fmaf(sn, tu[o + HS + 1], fmaf(cs, tq[o + HS + 1], t11));
+    const float4 a = tile[o], b = tile[o + 1], c = tile[o + HS], d = tile[o + HS + 1];
+    const float t00 = HasResp ? a.x * tr : a.x, t01 = HasResp ? b.x * tr : b.x;
+    const float t10 = HasResp ? c.x * tr : c.x, t11 = HasResp ? d.x * tr : d.x;
+    const float v00 = fmaf(sn, a.z, fmaf(cs, a.y, t00));
+    const float v01 = fmaf(sn, b.z, fmaf(cs, b.y, t01));
+    const float v10 = fmaf(sn, c.z, fmaf(cs, c.y, t10));
+    const float v11 = fmaf(sn, d.z, fmaf(cs, d.y, t11));
     const float top = fmaf(dx, v01 - v00, v00);
     const float bot = fmaf(dx, v11 - v10, v10);
     return fmaf(dy, bot - top, top);
@@ -390,7 +432,9 @@ __device__ inline float m2t_interp(const float *__restrict__ tile, int o, float 
 
 // ------------------------------------------------------------------------------------------
 // sin and cos of 2*alpha from a 256-entry table in LDS: x = k * (2 pi / 256) + d with
-// k = rint(x * 256 / (2 pi)) and d from a three-float Cody-Waite split of 2 pi / 256, then
+// k = rint(x * 256 / (2 pi)) and d from a two-float Cody-Waite split of 2 pi / 256 (the first
+// product is exact inside the FMA; the omitted third part, 2.8e-17, shifts d by at most
+// |k| * 2.8e-17 < 2.3e-13 for |x| <= 2e5, far below one ulp of sin or cos), then
 // sin x = S_k cos d + C_k sin d, cos x = C_k cos d - S_k sin d with cos d = 1 - d^2/2 and
 // sin d = d (1 - d^2/6). For |x| <= 2e5 the product x * 256/(2 pi) is below 2^23, so k is within
 // 0.75 of the exact quotient and |d| <= 0.0185, where the dropped terms (d^4/24 < 5e-9,
@@ -412,8 +456,7 @@ __device__ inline void sincos_tab(const float2 *__restrict__ tab, float x, float
 {
     const float k = rintf(x * 40.7436638f);              // 256 / (2 pi)
     float d = fmaf(k, -0.024543693f, x);                  // 2 pi / 256 = 0.024543693
-    d = fmaf(k, 6.8299044e-10f, d);                       //            - 6.8299044e-10
-    d = fmaf(k, 2.7755576e-17f, d);                       //            - 2.7755576e-17
+    d = fmaf(k, 6.8299044e-10f, d);                       //            - 6.8299044e-10 (+ 2.8e-17)
     const float2 t = tab[int(k) & (SCT - 1)];
     const float z = d * d;
     const float cd = fmaf(z, -0.5f, 1.0f);
@@ -441,7 +484,7 @@ __device__ inline bool quad_small_angles(const float4 &a4)
 }
 
 template<bool Periodic, bool HasResp, bool Small>
-__device__ inline float m2t_sample(const float *__restrict__ tile, const float2 *__restrict__ tab, float y, float x,
+__device__ inline float m2t_sample(const float4 *__restrict__ tile, const float2 *__restrict__ tab, float y, float x,
                                    float alpha, const Geom &g, int ybase, int xbase, float tr, float pr, Checks &chk,
                                    bool &owned)
 {
@@ -462,7 +505,7 @@ __device__ inline float m2t_sample(const float *__restrict__ tile, const float2 
 template<bool Periodic, bool Vec, bool HasResp, typename TT>
 __device__ inline void m2t_entries(TT *__restrict__ tod, const float *__restrict__ xp,
                                    const uint64_t *__restrict__ plan_mt, const uint8_t *__restrict__ qmask,
-                                   int64_t nsamp, uint32_t wb, uint32_t we, const float *__restrict__ tile,
+                                   int64_t nsamp, uint32_t wb, uint32_t we, const float4 *__restrict__ tile,
                                    const float2 *__restrict__ sct, const Geom &g, int ybase, int xbase, bool accum,
                                    const Gains &gn, Checks &chk, uint32_t &err)
 {
@@ -497,16 +540,23 @@ __device__ inline void m2t_entries(TT *__restrict__ tod, const float *__restrict
 
             bool o0, o1, o2, o3;
             float4 v;
-            // Scheduling barriers keep the four samples from being interleaved, which would
-            // exceed the register budget (96 VGPRs, 5 waves per SIMD). (A wave-uniform split into
-            // a small-angle path, as in tod2map, costs this kernel registers and spills.)
-            v.x = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.x, x4.x, a4.x, g, ybase, xbase, tr[0], pr[0], chk, o0);
-            __builtin_amdgcn_sched_barrier(0);
-            v.y = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.y, x4.y, a4.y, g, ybase, xbase, tr[1], pr[1], chk, o1);
-            __builtin_amdgcn_sched_barrier(0);
-            v.z = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.z, x4.z, a4.z, g, ybase, xbase, tr[2], pr[2], chk, o2);
-            __builtin_amdgcn_sched_barrier(0);
-            v.w = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.w, x4.w, a4.w, g, ybase, xbase, tr[3], pr[3], chk, o3);
+            if (quad_small_angles(a4)) {
+                // Every angle of the wave's groups takes the table path: straight-line code, so
+                // the LDS reads of the four samples overlap.
+                v.x = m2t_sample<Periodic, HasResp, true>(tile, sct, y4.x, x4.x, a4.x, g, ybase, xbase, tr[0], pr[0], chk, o0);
+                v.y = m2t_sample<Periodic, HasResp, true>(tile, sct, y4.y, x4.y, a4.y, g, ybase, xbase, tr[1], pr[1], chk, o1);
+                v.z = m2t_sample<Periodic, HasResp, true>(tile, sct, y4.z, x4.z, a4.z, g, ybase, xbase, tr[2], pr[2], chk, o2);
+                v.w = m2t_sample<Periodic, HasResp, true>(tile, sct, y4.w, x4.w, a4.w, g, ybase, xbase, tr[3], pr[3], chk, o3);
+            }
+            else {
+                v.x = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.x, x4.x, a4.x, g, ybase, xbase, tr[0], pr[0], chk, o0);
+                __builtin_amdgcn_sched_barrier(0);
+                v.y = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.y, x4.y, a4.y, g, ybase, xbase, tr[1], pr[1], chk, o1);
+                __builtin_amdgcn_sched_barrier(0);
+                v.z = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.z, x4.z, a4.z, g, ybase, xbase, tr[2], pr[2], chk, o2);
+                __builtin_amdgcn_sched_barrier(0);
+                v.w = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.w, x4.w, a4.w, g, ybase, xbase, tr[3], pr[3], chk, o3);
+            }
             if constexpr (!std::is_same<TT, float>::value) {
                 // The hardware reciprocal (1 ulp) is exact enough ahead of rounding to 16 bits.
                 v.x *= __builtin_amdgcn_rcpf(sc[0]);
@@ -534,17 +584,17 @@ __device__ inline void m2t_entries(TT *__restrict__ tod, const float *__restrict
 }
 
 template<bool Periodic, bool Vec, bool HasResp, typename TT>
-__global__ void __launch_bounds__(M2T_THREADS, 5)
+__global__ void __launch_bounds__(M2T_THREADS, 4)
 planned_map2tod_kernel(TT *__restrict__ tod, const float *__restrict__ lmap, const float *__restrict__ xp,
                        const int64_t *__restrict__ cell_offsets, const uint64_t *__restrict__ plan_mt,
                        const Chunk *__restrict__ chunks, const uint8_t *__restrict__ qmask,
                        uint32_t *__restrict__ err_word, int64_t nsamp, int ny, int nx, int nycells, int nxcells,
                        int64_t ystride, int64_t polstride, bool partial, bool accum, Gains gn)
 {
-    __shared__ float tile[3 * HPLANE];
+    __shared__ float4 tile[HPLANE];                       // 66 KiB
     __shared__ float2 sct[SCT];
 
-    const Chunk ch = chunks[blockIdx.x];
+    const Chunk ch = chunks[xcd_chunk(blockIdx.x, gridDim.x)];
     const HaloCells h = halo_cells(ch.icell, cell_offsets, nycells, nxcells, nx, Periodic);
     uint32_t err = (h.own < 0 && !partial) ? ERR_NOT_IN_PIXELIZATION : 0u;
 
@@ -561,7 +611,7 @@ planned_map2tod_kernel(TT *__restrict__ tod, const float *__restrict__ lmap, con
     wave_slice(ch, wave, M2T_THREADS / WAVE, wb, we);
     m2t_entries<Periodic, Vec, HasResp, TT>(tod, xp, plan_mt, qmask, nsamp, wb, we, tile, sct, g, ybase, xbase, accum,
                                             gn, chk, err);
-    report_err(err | chk.bits(), err_word);
+    report_err(err | chk.bits(g), err_word);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -815,7 +865,7 @@ __device__ inline void t2m_flush(float *__restrict__ lmap, const unsigned long l
 // and its inverse scale; ends synchronized.
 template<int NT, typename Run>
 __device__ inline bool t2m_accumulate(unsigned long long *__restrict__ acc, const uint32_t *s_max, uint32_t *s_redo,
-                                      Run &&run, float &inv_scale, uint32_t &err, int tid)
+                                      const Geom &g, Run &&run, float &inv_scale, uint32_t &err, int tid)
 {
     const int lane = threadIdx.x & 63;
     uint32_t mbits = *s_max;
@@ -830,7 +880,7 @@ __device__ inline bool t2m_accumulate(unsigned long long *__restrict__ acc, cons
         float tsum = 0.0f;                               // NaN iff some deposited t is NaN
         Checks chk;
         run(finite, scale, mrun, tsum, chk);
-        err |= chk.bits();
+        err |= chk.bits(g);
         if (!finite)
             break;
         const bool lane_nan = !(tsum == tsum);
@@ -868,7 +918,7 @@ planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, con
     float *facc = reinterpret_cast<float *>(acc);         // float tile for the non-finite fallback
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const Chunk ch = chunks[blockIdx.x];
+    const Chunk ch = chunks[xcd_chunk(blockIdx.x, gridDim.x)];
     const HaloCells h = halo_cells(ch.icell, cell_offsets, nycells, nxcells, nx, Periodic);
     uint32_t err = (h.own < 0 && !partial) ? ERR_NOT_IN_PIXELIZATION : 0u;
 
@@ -902,7 +952,7 @@ planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, con
 
     float inv_scale;
     const bool finite = t2m_accumulate<T2M_THREADS>(
-        acc, &s_max, s_redo,
+        acc, &s_max, s_redo, g,
         [&](bool fin, float scale, float &mrun, float &tsum, Checks &chk) {
             t2m_entries<Periodic, Vec, HasResp, TT>(acc, facc, tod, xp, plan_mt, qmask, nsamp, wb, we, sct, g, ybase,
                                                     xbase, gn, fin, scale, mrun, tsum, chk, err);
@@ -994,12 +1044,12 @@ __device__ inline Gains batch_gains(const BatchTod &td)
 }
 
 template<bool Periodic, typename TT, int NT>
-__global__ void __launch_bounds__(NT, 5)
+__global__ void __launch_bounds__(NT, 4)
 batched_map2tod_kernel(const float *__restrict__ lmap, const BatchTod *__restrict__ tods,
                        const BatchSeg *__restrict__ segs, const BatchItem *__restrict__ items,
                        const int64_t *__restrict__ cell_offsets, BatchArgs ba)
 {
-    __shared__ float tile[3 * HPLANE];
+    __shared__ float4 tile[HPLANE];                       // 66 KiB
     __shared__ float2 sct[SCT];
     __shared__ uint32_t s_item;
 
@@ -1034,7 +1084,7 @@ batched_map2tod_kernel(const float *__restrict__ lmap, const BatchTod *__restric
             break;
     }
     batch_finish(ba);
-    report_err(err | chk.bits(), ba.err_word);
+    report_err(err | chk.bits(g), ba.err_word);
 }
 
 template<bool Periodic, typename TT>
@@ -1093,7 +1143,7 @@ batched_tod2map_kernel(float *__restrict__ lmap, const BatchTod *__restrict__ to
 
         float inv_scale;
         const bool finite = t2m_accumulate<T2M_THREADS>(
-            acc, &s_max, s_redo,
+            acc, &s_max, s_redo, g,
             [&](bool fin, float scale, float &mrun, float &tsum, Checks &chk) {
                 batch_wave_segments(segs, it, wb, we, [&](const BatchSeg &sg, uint32_t b, uint32_t e) {
                     const BatchTod td = tods[sg.tod];
