@@ -373,24 +373,45 @@ __device__ inline HaloCells halo_cells(uint32_t icell, const int64_t *__restrict
 }
 
 // map2tod's tile holds each pixel's (T, Q, U) together in one float4 (the fourth lane unused),
-// so the four corners of a sample are four 16-byte LDS reads from one base address.
+// so the four corners of a sample are four 16-byte LDS reads from one base address. The
+// interior is read as float4 groups of 4 pixels of a row from each plane (cell offsets, ystride
+// and polstride are multiples of 64 floats, so a 16-byte aligned map keeps every group aligned),
+// all of a thread's groups loaded before any is written, so the loads overlap.
 template<int NT>
 __device__ inline void stage_halo_tile(float4 *__restrict__ tile, const float *__restrict__ lmap, const HaloCells &h,
                                        int64_t ystride, int64_t polstride, int tid)
 {
-    // Interior (column W excluded when W < 64: it holds the wrapped column).
-    for (int i = tid; i < CELL_NPIX; i += NT) {
-        const int r = i >> 6, c = i & 63;
-        if (c == h.w)
-            continue;
-        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    constexpr int NG = CELL_NPIX / 4 / NT;            // groups of 4 pixels per thread
+    static_assert(NG * 4 * NT == CELL_NPIX, "block size must divide the cell");
+    const bool aligned = (reinterpret_cast<uintptr_t>(lmap) & 15) == 0;
+    float4 t[NG], q[NG], u[NG];
+#pragma unroll
+    for (int k = 0; k < NG; k++) {
+        const int g = tid + k * NT, r = g >> 4, c0 = (g & 15) << 2;
+        t[k] = q[k] = u[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (h.own >= 0) {
-            const float *p = lmap + h.own + r * ystride + c;
-            v.x = p[0];
-            v.y = p[polstride];
-            v.z = p[2 * polstride];
+            const float *p = lmap + h.own + r * ystride + c0;
+            if (aligned) {
+                t[k] = *reinterpret_cast<const float4 *>(p);
+                q[k] = *reinterpret_cast<const float4 *>(p + polstride);
+                u[k] = *reinterpret_cast<const float4 *>(p + 2 * polstride);
+            }
+            else {
+                t[k] = make_float4(p[0], p[1], p[2], p[3]);
+                q[k] = make_float4(p[polstride], p[polstride + 1], p[polstride + 2], p[polstride + 3]);
+                u[k] = make_float4(p[2 * polstride], p[2 * polstride + 1], p[2 * polstride + 2], p[2 * polstride + 3]);
+            }
         }
-        tile[r * HS + c] = v;
+    }
+#pragma unroll
+    for (int k = 0; k < NG; k++) {
+        const int g = tid + k * NT, r = g >> 4, c0 = (g & 15) << 2;
+        float4 *row = tile + r * HS + c0;
+        // Column W (when W < 64) holds the wrapped column and is written below.
+        if (c0 + 0 != h.w) row[0] = make_float4(t[k].x, q[k].x, u[k].x, 0.0f);
+        if (c0 + 1 != h.w) row[1] = make_float4(t[k].y, q[k].y, u[k].y, 0.0f);
+        if (c0 + 2 != h.w) row[2] = make_float4(t[k].z, q[k].z, u[k].z, 0.0f);
+        if (c0 + 3 != h.w) row[3] = make_float4(t[k].w, q[k].w, u[k].w, 0.0f);
     }
     // Row 64 (columns 0..W) and column W (rows 0..63).
     const int nrow = h.w + 1, nper = nrow + 64;
@@ -798,65 +819,101 @@ __device__ inline int t2m_scale_exp(uint32_t mbits)
     return clamp_i(27 - (int(mbits >> 23) - 127), -100, 126);
 }
 
-// Clears the halo tile: all of it for the fixed-point form, the float view's part otherwise.
+// Clears the halo tile (16 bytes per store): all of it for the fixed-point form, the float
+// view's part (the first half) otherwise. The tile is 16-byte aligned.
 template<int NT>
 __device__ inline void t2m_clear(unsigned long long *__restrict__ acc, bool finite, int tid)
 {
-    for (int i = tid; i < 3 * HPLANE; i += NT) {
-        if (finite)
-            acc[i] = 0ull;
-        else if (i < (3 * HPLANE + 1) / 2)
-            acc[i] = 0ull;        // the float tile is the first half
-    }
+    constexpr int NPAIR = (3 * HPLANE + 1) / 2;
+    ulonglong2 *a2 = reinterpret_cast<ulonglong2 *>(acc);
+    const int n = finite ? (3 * HPLANE) / 2 : (NPAIR + 1) / 2;
+    for (int i = tid; i < n; i += NT)
+        a2[i] = make_ulonglong2(0ull, 0ull);
+    if (finite && tid == 0)
+        acc[3 * HPLANE - 1] = 0ull;               // 3 * HPLANE is odd
+}
+
+__device__ inline float t2m_value(const unsigned long long *__restrict__ acc, const float *__restrict__ facc, int k,
+                                  bool finite, float inv_scale)
+{
+    return finite ? float((long long)acc[k]) * inv_scale : facc[k];
 }
 
 // Flush of the halo tile into the map. The cell's row 0 and column 0 also receive other
 // blocks' halo deposits (and column 0 of cell column 0 the wrapped ones), so they and every
-// split cell are added atomically; the rest of the cell with a plain read-modify-write. Halo row
-// 64 and column W go to the neighbouring cells, atomically; deposits into absent cells are
-// dropped.
+// split cell are added atomically; the rest of the cell with a plain read-modify-write, as
+// float4 groups of 4 pixels of a row where the whole group allows it (all of a thread's groups
+// loaded before any is stored). Halo row 64 and column W go to the neighbouring cells,
+// atomically; deposits into absent cells are dropped.
 template<int NT>
 __device__ inline void t2m_flush(float *__restrict__ lmap, const unsigned long long *__restrict__ acc,
                                  const float *__restrict__ facc, const HaloCells &h, bool shared, bool finite,
                                  float inv_scale, int64_t ystride, int64_t polstride, int tid)
 {
-    const int nrow = h.w + 1, nper = nrow + 64;
-    for (int i = tid; i < 3 * (CELL_NPIX + nper); i += NT) {
-        int pl, k;
-        int64_t dst;
-        bool atomic = true;
-        if (i < 3 * CELL_NPIX) {
-            pl = i >> 12;
-            const int r = (i >> 6) & 63, c = i & 63;
-            if (c == h.w || h.own < 0)
-                continue;
-            k = r * HS + c;
-            dst = h.own + r * ystride + c;
-            atomic = shared || r == 0 || c == 0;
+    constexpr int NGRP = 3 * CELL_NPIX / 4;           // groups of 4 pixels, all planes
+    constexpr int NG = (NGRP + NT - 1) / NT;
+    const bool aligned = (reinterpret_cast<uintptr_t>(lmap) & 15) == 0;
+    if (h.own >= 0) {
+        float4 old[NG];
+        bool vec[NG];
+#pragma unroll
+        for (int k = 0; k < NG; k++) {
+            const int g = tid + k * NT;
+            const int pl = g >> 10, r = (g >> 4) & 63, c0 = (g & 15) << 2;
+            vec[k] = (g < NGRP) && aligned && !shared && r != 0 && c0 != 0 && (h.w < c0 || h.w > c0 + 3);
+            if (vec[k])
+                old[k] = *reinterpret_cast<const float4 *>(lmap + h.own + pl * polstride + r * ystride + c0);
         }
-        else {
-            const int i2 = i - 3 * CELL_NPIX;
-            pl = i2 / nper;
-            const int jj = i2 - pl * nper;
-            if (jj < nrow) {
-                k = 64 * HS + jj;
-                dst = (jj < h.w) ? ((h.down >= 0) ? h.down + jj : -1) : h.diag;
+#pragma unroll
+        for (int k = 0; k < NG; k++) {
+            const int g = tid + k * NT;
+            if (g >= NGRP)
+                continue;
+            const int pl = g >> 10, r = (g >> 4) & 63, c0 = (g & 15) << 2;
+            const int kt = pl * HPLANE + r * HS + c0;
+            float *gp = lmap + h.own + pl * polstride + r * ystride + c0;
+            if (vec[k]) {
+                float4 v = old[k];
+                v.x += t2m_value(acc, facc, kt, finite, inv_scale);
+                v.y += t2m_value(acc, facc, kt + 1, finite, inv_scale);
+                v.z += t2m_value(acc, facc, kt + 2, finite, inv_scale);
+                v.w += t2m_value(acc, facc, kt + 3, finite, inv_scale);
+                *reinterpret_cast<float4 *>(gp) = v;
             }
             else {
-                const int r = jj - nrow;
-                k = r * HS + h.w;
-                dst = (h.right >= 0) ? h.right + r * ystride : -1;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int c = c0 + j;
+                    if (c == h.w)
+                        continue;
+                    const float v = t2m_value(acc, facc, kt + j, finite, inv_scale);
+                    if (shared || r == 0 || c == 0)
+                        unsafeAtomicAdd(gp + j, v);
+                    else
+                        gp[j] += v;
+                }
             }
-            if (dst < 0)
-                continue;
         }
-        k += pl * HPLANE;
-        const float v = finite ? float((long long)acc[k]) * inv_scale : facc[k];
-        float *gp = lmap + dst + pl * polstride;
-        if (atomic)
-            unsafeAtomicAdd(gp, v);
-        else
-            *gp += v;
+    }
+    // Halo row 64 (columns 0..W) and column W (rows 0..63), to the neighbouring cells.
+    const int nrow = h.w + 1, nper = nrow + 64;
+    for (int i = tid; i < 3 * nper; i += NT) {
+        const int pl = i / nper;
+        const int jj = i - pl * nper;
+        int k;
+        int64_t dst;
+        if (jj < nrow) {
+            k = 64 * HS + jj;
+            dst = (jj < h.w) ? ((h.down >= 0) ? h.down + jj : -1) : h.diag;
+        }
+        else {
+            const int r = jj - nrow;
+            k = r * HS + h.w;
+            dst = (h.right >= 0) ? h.right + r * ystride : -1;
+        }
+        if (dst < 0)
+            continue;
+        unsafeAtomicAdd(lmap + dst + pl * polstride, t2m_value(acc, facc, pl * HPLANE + k, finite, inv_scale));
     }
 }
 
@@ -920,7 +977,7 @@ planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, con
                        uint32_t *__restrict__ err_word, int64_t nsamp, int ny, int nx, int nycells, int nxcells,
                        int64_t ystride, int64_t polstride, bool partial, Gains gn)
 {
-    __shared__ unsigned long long acc[3 * HPLANE];       // 99 KiB
+    __shared__ __align__(16) unsigned long long acc[3 * HPLANE];       // 99 KiB
     __shared__ float2 sct[SCT];
     __shared__ uint32_t s_max, s_redo[3];
     float *facc = reinterpret_cast<float *>(acc);         // float tile for the non-finite fallback
@@ -1101,7 +1158,7 @@ batched_tod2map_kernel(float *__restrict__ lmap, const BatchTod *__restrict__ to
                        const BatchSeg *__restrict__ segs, const BatchItem *__restrict__ items,
                        const int64_t *__restrict__ cell_offsets, BatchArgs ba)
 {
-    __shared__ unsigned long long acc[3 * HPLANE];       // 99 KiB
+    __shared__ __align__(16) unsigned long long acc[3 * HPLANE];       // 99 KiB
     __shared__ float2 sct[SCT];
     __shared__ uint32_t s_max, s_redo[3], s_item;
     float *facc = reinterpret_cast<float *>(acc);
