@@ -841,27 +841,45 @@ __device__ inline float t2m_value(const unsigned long long *__restrict__ acc, co
 
 // Flush of the halo tile into the map. The cell's row 0 and column 0 also receive other
 // blocks' halo deposits (and column 0 of cell column 0 the wrapped ones), so they and every
-// split cell are added atomically; the rest of the cell with a plain read-modify-write, as
-// float4 groups of 4 pixels of a row where the whole group allows it (all of a thread's groups
-// loaded before any is stored). Halo row 64 and column W go to the neighbouring cells,
-// atomically; deposits into absent cells are dropped.
+// split cell are added atomically; the rest of the cell with a plain read-modify-write. A split
+// cell is flushed one pixel per lane (each atomic instruction then touches few cache lines); an
+// unsplit one as float4 groups of 4 pixels per thread where the group allows it.
+// Halo row 64 and column W go to the neighbouring cells, atomically; deposits into absent cells
+// are dropped.
 template<int NT>
 __device__ inline void t2m_flush(float *__restrict__ lmap, const unsigned long long *__restrict__ acc,
                                  const float *__restrict__ facc, const HaloCells &h, bool shared, bool finite,
                                  float inv_scale, int64_t ystride, int64_t polstride, int tid)
 {
-    constexpr int NGRP = 3 * CELL_NPIX / 4;           // groups of 4 pixels, all planes
-    constexpr int NG = (NGRP + NT - 1) / NT;
-    const bool aligned = (reinterpret_cast<uintptr_t>(lmap) & 15) == 0;
-    if (h.own >= 0) {
+    const bool vec = !shared && ((reinterpret_cast<uintptr_t>(lmap) & 15) == 0);
+    if (h.own >= 0 && !vec) {
+        // Whole cell, one pixel per lane; atomic where another block may add too.
+        for (int i = tid; i < 3 * CELL_NPIX; i += NT) {
+            const int pl = i >> 12, r = (i >> 6) & 63, c = i & 63;
+            if (c == h.w)
+                continue;
+            const float v = t2m_value(acc, facc, pl * HPLANE + r * HS + c, finite, inv_scale);
+            float *gp = lmap + h.own + pl * polstride + r * ystride + c;
+            if (shared || r == 0 || c == 0)
+                unsafeAtomicAdd(gp, v);
+            else
+                *gp += v;
+        }
+    }
+    else if (h.own >= 0) {
+        // Groups of 4 pixels of a row, all planes; a group holding row 0, column 0 or column W
+        // is done pixel by pixel in its thread, the others as float4 read-modify-writes. All of
+        // a thread's float4 loads are issued first.
+        constexpr int NGRP = 3 * CELL_NPIX / 4;
+        constexpr int NG = (NGRP + NT - 1) / NT;
         float4 old[NG];
-        bool vec[NG];
+        bool full[NG];
 #pragma unroll
         for (int k = 0; k < NG; k++) {
             const int g = tid + k * NT;
             const int pl = g >> 10, r = (g >> 4) & 63, c0 = (g & 15) << 2;
-            vec[k] = (g < NGRP) && aligned && !shared && r != 0 && c0 != 0 && (h.w < c0 || h.w > c0 + 3);
-            if (vec[k])
+            full[k] = (g < NGRP) && r != 0 && c0 != 0 && (h.w < c0 || h.w > c0 + 3);
+            if (full[k])
                 old[k] = *reinterpret_cast<const float4 *>(lmap + h.own + pl * polstride + r * ystride + c0);
         }
 #pragma unroll
@@ -872,7 +890,7 @@ __device__ inline void t2m_flush(float *__restrict__ lmap, const unsigned long l
             const int pl = g >> 10, r = (g >> 4) & 63, c0 = (g & 15) << 2;
             const int kt = pl * HPLANE + r * HS + c0;
             float *gp = lmap + h.own + pl * polstride + r * ystride + c0;
-            if (vec[k]) {
+            if (full[k]) {
                 float4 v = old[k];
                 v.x += t2m_value(acc, facc, kt, finite, inv_scale);
                 v.y += t2m_value(acc, facc, kt + 1, finite, inv_scale);
@@ -887,7 +905,7 @@ __device__ inline void t2m_flush(float *__restrict__ lmap, const unsigned long l
                     if (c == h.w)
                         continue;
                     const float v = t2m_value(acc, facc, kt + j, finite, inv_scale);
-                    if (shared || r == 0 || c == 0)
+                    if (r == 0 || c == 0)
                         unsafeAtomicAdd(gp + j, v);
                     else
                         gp[j] += v;
