@@ -102,9 +102,11 @@ void check_config(const gmm_batch_config &c, const char *where)
 
 // Work items over the cell runs: the entries of each cell (all TODs, in TOD order) are cut
 // into ceil(total / target) nearly equal items; items of a cell that is cut are flagged
-// CHUNK_SHARED (their tod2map flushes add atomically). Items are ordered by decreasing size,
-// so that a persistent grid taking them in order starts the long ones first.
-void build_items(const std::vector<CellRun> &runs, int64_t target, std::vector<BatchSeg> &segs,
+// CHUNK_SHARED (their tod2map flushes add atomically). Items stay in cell order for a grid of
+// one block per item (the kernel spreads runs of neighbouring cells over the XCDs, as the plain
+// operators do), and are ordered by decreasing size for a persistent grid, which takes them in
+// order and so starts the long ones first.
+void build_items(const std::vector<CellRun> &runs, int64_t target, bool by_size, std::vector<BatchSeg> &segs,
                  std::vector<BatchItem> &items)
 {
     segs.clear();
@@ -139,15 +141,16 @@ void build_items(const std::vector<CellRun> &runs, int64_t target, std::vector<B
         }
         g0 = g1;
     }
-    std::stable_sort(items.begin(), items.end(),
-                     [](const BatchItem &a, const BatchItem &b) { return a.nent > b.nent; });
+    if (by_size)
+        std::stable_sort(items.begin(), items.end(),
+                         [](const BatchItem &a, const BatchItem &b) { return a.nent > b.nent; });
 }
 
 void upload_tables(gmm_batch &b, DeviceTables &t, int64_t target, int blocks_per_cu, int threads)
 {
     std::vector<BatchSeg> segs;
     std::vector<BatchItem> items;
-    build_items(b.runs, target, segs, items);
+    build_items(b.runs, target, blocks_per_cu > 0, segs, items);
     if (segs.size() >= (size_t(1) << 32) || items.size() >= (size_t(1) << 31))
         fail("batch: too many work items");
     t.release();

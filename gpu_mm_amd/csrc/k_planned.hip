@@ -1070,12 +1070,13 @@ struct BatchArgs
 };
 
 // Item of the block's next work unit, the same in every lane: the next one from the queue
-// (persistent grid) or the block's own. Ends synchronized. (Other threads may reset
-// block-shared words before the call: the synchronization covers them.)
+// (persistent grid) or the block's own, with runs of neighbouring items on one XCD as for the
+// plain operators. Ends synchronized. (Other threads may reset block-shared words before the
+// call: the synchronization covers them.)
 __device__ inline uint32_t batch_next_item(const BatchArgs &ba, uint32_t *s_item)
 {
     if (!ba.persistent)
-        return blockIdx.x;
+        return xcd_chunk(blockIdx.x, gridDim.x);
     if (threadIdx.x == 0)
         *s_item = atomicAdd(ba.counters, 1u);
     __syncthreads();
