@@ -308,6 +308,7 @@ int gmm_batch_create(gmm_batch **out, const gmm_lpix *lpix, int32_t ntod, const 
             require_ptr(xpointing_gpu[i], ws.c_str(), "xpointing");
             require_ptr(tod_gpu[i], ws.c_str(), "tod");
             check_nsamp(nsamp[i], ws.c_str());
+            check_nsamp_quads(nsamp[i], ws.c_str());
             if (pl->pp.nsamp != nsamp[i])
                 fail(ws + ": plan was built for nsamp=" + std::to_string(pl->pp.nsamp) + ", got "
                      + std::to_string(nsamp[i]));

@@ -104,6 +104,7 @@ void throw_errflags(uint32_t err, const char *where, uint32_t ignore = 0);
 
 // Argument checks (src_lib/check_arguments.cu:137-156).
 void check_nsamp(int64_t nsamp, const char *where);
+void check_nsamp_quads(int64_t nsamp, const char *where);
 void check_npix_global(int64_t n, const char *what, const char *where);
 
 // ------------------------------------------------------------------------------------------

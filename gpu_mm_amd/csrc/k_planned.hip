@@ -111,8 +111,10 @@ __device__ inline uint32_t pack_entry(uint64_t mt)
 
 // ------------------------------------------------------------------------------------------
 // Lane layout of both kernels: a wave step covers 8 plan entries (lines); lane = 8*li + q takes
-// samples 4q..4q+3 of line li, loaded as float4 (the arrays are 16-byte aligned in the Vec
-// variant; 32-sample lines make every 4-sample group aligned).
+// samples 4q..4q+3 of line li, loaded as float4 (the Vec variant requires 16-byte aligned arrays
+// and nsamp % 4 == 0, which keep every 4-sample group of every row aligned). The last line of a
+// TOD may be partial: its groups past the end have no owned-quad bit, and a group cut by the end
+// (nsamp % 4 != 0, scalar variant only) reads zeros past it and writes nothing there.
 
 template<bool Vec>
 __device__ inline float4 load4(const float *__restrict__ p)
@@ -120,6 +122,16 @@ __device__ inline float4 load4(const float *__restrict__ p)
     if (Vec)
         return *reinterpret_cast<const float4 *>(p);
     return make_float4(p[0], p[1], p[2], p[3]);
+}
+
+// The first nv (1..4) of 4 consecutive floats, zeros after: the last group of a TOD whose length
+// is not a multiple of 4 (only in the scalar variant: the vector one requires nsamp % 4 == 0).
+template<bool Vec>
+__device__ inline float4 load4n(const float *__restrict__ p, uint32_t nv)
+{
+    if (Vec || nv >= 4)
+        return load4<Vec>(p);
+    return make_float4(p[0], (nv > 1) ? p[1] : 0.0f, (nv > 2) ? p[2] : 0.0f, 0.0f);
 }
 
 
@@ -147,6 +159,15 @@ struct TodIO
             const v4 v = *reinterpret_cast<const v4 *>(p);
             return make_float4(float(v.x), float(v.y), float(v.z), float(v.w));
         }
+    }
+
+    // The first nv samples of the group (16-bit TODs have a length divisible by 4: nv == 4).
+    __device__ static float4 load_n(const TT *__restrict__ p, uint32_t nv)
+    {
+        if constexpr (std::is_same<TT, float>::value)
+            return load4n<Vec>(p, nv);
+        else
+            return load(p);
     }
 
     __device__ static void store(TT *__restrict__ p, float4 v)
@@ -198,30 +219,31 @@ struct Gains
 };
 
 // Row of each of the 4 samples s0..s0+3 for rows of 'nper' samples (one division per quad when
-// a quad can cross at most one row boundary).
-__device__ inline void quad_rows(uint32_t s0, uint32_t nper, uint32_t (&r)[4])
+// a quad can cross at most one row boundary). Only the first nv samples exist (nv < 4 for the
+// last group of a TOD whose length is not a multiple of 4): the others take the row of the last.
+__device__ inline void quad_rows(uint32_t s0, uint32_t nper, uint32_t (&r)[4], uint32_t nv = 4)
 {
     if (nper >= 4) {
         const uint32_t r0 = s0 / nper;
         const uint32_t b = (r0 + 1) * nper - s0;    // samples of the quad in row r0
 #pragma unroll
         for (int k = 0; k < 4; k++)
-            r[k] = r0 + ((uint32_t(k) >= b) ? 1u : 0u);
+            r[k] = r0 + ((min(uint32_t(k), nv - 1) >= b) ? 1u : 0u);
     }
     else {
 #pragma unroll
         for (int k = 0; k < 4; k++)
-            r[k] = (s0 + uint32_t(k)) / nper;
+            r[k] = (s0 + min(uint32_t(k), nv - 1)) / nper;
     }
 }
 
 // T and P gain of the 4 samples of a quad (1 without a response array).
 template<bool HasResp>
-__device__ inline void quad_resp(const Gains &g, uint32_t s0, float (&tr)[4], float (&pr)[4])
+__device__ inline void quad_resp(const Gains &g, uint32_t s0, float (&tr)[4], float (&pr)[4], uint32_t nv = 4)
 {
     if (HasResp) {
         uint32_t d[4];
-        quad_rows(s0, g.resp_nper, d);
+        quad_rows(s0, g.resp_nper, d, nv);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             tr[k] = g.resp[d[k]];
@@ -235,7 +257,7 @@ __device__ inline void quad_resp(const Gains &g, uint32_t s0, float (&tr)[4], fl
     }
 }
 
-// Scale of the 4 samples of a quad (16-bit TODs only).
+// Scale of the 4 samples of a quad (16-bit TODs only; their length is a multiple of 4).
 template<typename TT>
 __device__ inline void quad_scale(const Gains &g, uint32_t s0, float (&sc)[4])
 {
@@ -540,7 +562,7 @@ __device__ inline void m2t_entries(TT *__restrict__ tod, const float *__restrict
 {
     const int lane = threadIdx.x & 63;
     const int li = lane >> 3, q = lane & 7;
-    const uint32_t ncl = uint32_t(nsamp >> 5);
+    const uint32_t ncl = uint32_t((nsamp + 31) >> 5);     // the last line may be partial
     using IO = TodIO<TT, Vec>;
 
     for (uint32_t base = wb; base < we; base += 64) {
@@ -560,11 +582,12 @@ __device__ inline void m2t_entries(TT *__restrict__ tod, const float *__restrict
             if (!act || !((qm >> q) & 1u))
                 continue;
             const int64_t s0 = int64_t(icl) * LINE + 4 * q;
-            const float4 y4 = load4<Vec>(xp + s0);
-            const float4 x4 = load4<Vec>(xp + nsamp + s0);
-            const float4 a4 = load4<Vec>(xp + 2 * nsamp + s0);
+            const uint32_t nv = uint32_t(min<int64_t>(4, nsamp - s0));   // < 4 only at the end of a TOD
+            const float4 y4 = load4n<Vec>(xp + s0, nv);
+            const float4 x4 = load4n<Vec>(xp + nsamp + s0, nv);
+            const float4 a4 = load4n<Vec>(xp + 2 * nsamp + s0, nv);
             float tr[4], pr[4], sc[4];
-            quad_resp<HasResp>(gn, uint32_t(s0), tr, pr);
+            quad_resp<HasResp>(gn, uint32_t(s0), tr, pr, nv);
             quad_scale<TT>(gn, uint32_t(s0), sc);
 
             bool o0, o1, o2, o3;
@@ -586,6 +609,10 @@ __device__ inline void m2t_entries(TT *__restrict__ tod, const float *__restrict
                 __builtin_amdgcn_sched_barrier(0);
                 v.w = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.w, x4.w, a4.w, g, ybase, xbase, tr[3], pr[3], chk, o3);
             }
+            // Samples past the end of the TOD (read as zeros) are not written.
+            o1 = o1 && nv > 1;
+            o2 = o2 && nv > 2;
+            o3 = o3 && nv > 3;
             if constexpr (!std::is_same<TT, float>::value) {
                 // The hardware reciprocal (1 ulp) is exact enough ahead of rounding to 16 bits.
                 v.x *= __builtin_amdgcn_rcpf(sc[0]);
@@ -706,16 +733,18 @@ __device__ inline float t2m_probe(const TT *__restrict__ tod, const uint64_t *__
     using IO = TodIO<TT, Vec>;
     const int lane = threadIdx.x & 63;
     const int li = lane >> 3, q = lane & 7;
-    const uint32_t ncl = uint32_t(nsamp >> 5);
+    const uint32_t ncl = uint32_t((nsamp + 31) >> 5);     // the last line may be partial
     float m = 0.0f;
     const uint32_t idx = wb + uint32_t(li);
     if (idx < we) {
         const uint32_t icl = uint32_t(plan_mt[idx] >> ICL_SHIFT) & ICL_MASK;
-        if (icl < ncl) {
-            const int64_t s0 = int64_t(icl) * LINE + 4 * q;
-            const float4 t4 = IO::load(tod + s0);
+        const int64_t s0 = int64_t(icl) * LINE + 4 * q;
+        // Groups past the end of a partial last line hold no sample (nothing to read).
+        if (icl < ncl && s0 < nsamp) {
+            const uint32_t nv = uint32_t(min<int64_t>(4, nsamp - s0));
+            const float4 t4 = IO::load_n(tod + s0, nv);
             float tr[4], pr[4], sc[4];
-            quad_resp<HasResp>(gn, uint32_t(s0), tr, pr);
+            quad_resp<HasResp>(gn, uint32_t(s0), tr, pr, nv);
             quad_scale<TT>(gn, uint32_t(s0), sc);
             const float tv[4] = { t4.x, t4.y, t4.z, t4.w };
 #pragma unroll
@@ -744,7 +773,7 @@ __device__ inline void t2m_entries(unsigned long long *__restrict__ acc, float *
     using IO = TodIO<TT, Vec>;
     const int lane = threadIdx.x & 63;
     const int li = lane >> 3, q = lane & 7;
-    const uint32_t ncl = uint32_t(nsamp >> 5);
+    const uint32_t ncl = uint32_t((nsamp + 31) >> 5);     // the last line may be partial
 
     for (uint32_t base = wb; base < we; base += 64) {
         const uint32_t cnt = min(64u, we - base);
@@ -762,12 +791,14 @@ __device__ inline void t2m_entries(unsigned long long *__restrict__ acc, float *
             if (!act || !((qm >> q) & 1u))
                 continue;
             const int64_t s0 = int64_t(icl) * LINE + 4 * q;
-            const float4 y4 = load4<Vec>(xp + s0);
-            const float4 x4 = load4<Vec>(xp + nsamp + s0);
-            const float4 a4 = load4<Vec>(xp + 2 * nsamp + s0);
-            float4 t4 = IO::load(tod + s0);
+            // Past the end of the TOD (nv < 4) the samples read as zeros: they deposit nothing.
+            const uint32_t nv = uint32_t(min<int64_t>(4, nsamp - s0));
+            const float4 y4 = load4n<Vec>(xp + s0, nv);
+            const float4 x4 = load4n<Vec>(xp + nsamp + s0, nv);
+            const float4 a4 = load4n<Vec>(xp + 2 * nsamp + s0, nv);
+            float4 t4 = IO::load_n(tod + s0, nv);
             float tr[4], pr[4], sc[4];
-            quad_resp<HasResp>(gn, uint32_t(s0), tr, pr);
+            quad_resp<HasResp>(gn, uint32_t(s0), tr, pr, nv);
             quad_scale<TT>(gn, uint32_t(s0), sc);
             if (Q16) {
                 t4.x *= sc[0];
@@ -1254,7 +1285,7 @@ void launch_m2t(TT *tod, const float *lmap, const float *xp, const OpArgs &a, co
     const gmm_lpix &lp = *a.lpix;
     uint32_t *err_word = plan.err + (plan.pp.n_err - 1);
     dim3 grid(unsigned(plan.nchunks)), block(M2T_THREADS);
-    const bool vec = ((reinterpret_cast<uintptr_t>(tod) | reinterpret_cast<uintptr_t>(xp)) & 15) == 0;
+    const bool vec = (((reinterpret_cast<uintptr_t>(tod) | reinterpret_cast<uintptr_t>(xp)) & 15) == 0) && (a.nsamp % 4 == 0);
     auto k = lp.periodic_xcoord ? (vec ? planned_map2tod_kernel<true, true, HasResp, TT> : planned_map2tod_kernel<true, false, HasResp, TT>)
                                 : (vec ? planned_map2tod_kernel<false, true, HasResp, TT> : planned_map2tod_kernel<false, false, HasResp, TT>);
     kernel_timer_mark(TK_MAP2TOD, false, stream);
@@ -1272,7 +1303,7 @@ void launch_t2m(float *lmap, const TT *tod, const float *xp, const OpArgs &a, co
     const gmm_lpix &lp = *a.lpix;
     uint32_t *err_word = plan.err + (plan.pp.n_err - 1);
     dim3 grid(unsigned(plan.nchunks)), block(T2M_THREADS);
-    const bool vec = ((reinterpret_cast<uintptr_t>(tod) | reinterpret_cast<uintptr_t>(xp)) & 15) == 0;
+    const bool vec = (((reinterpret_cast<uintptr_t>(tod) | reinterpret_cast<uintptr_t>(xp)) & 15) == 0) && (a.nsamp % 4 == 0);
     auto k = lp.periodic_xcoord ? (vec ? planned_tod2map_kernel<true, true, HasResp, TT> : planned_tod2map_kernel<true, false, HasResp, TT>)
                                 : (vec ? planned_tod2map_kernel<false, true, HasResp, TT> : planned_tod2map_kernel<false, false, HasResp, TT>);
     kernel_timer_mark(TK_TOD2MAP, false, stream);

@@ -132,8 +132,8 @@ preplan_count_kernel(const float *__restrict__ xp, int64_t nsamp, int64_t ncl, i
 #pragma unroll
         for (int u = 0; u < PP_UNROLL; u++) {
             int64_t line = lb + 2 * NW * u + half;
-            act[u] = (line < l1);
             int64_t s = line * LINE + sub;
+            act[u] = (line < l1) && (s < nsamp);        // the last line may be partial
             y[u] = act[u] ? xp[s] : 0.0f;
             x[u] = act[u] ? xp[s + nsamp] : 0.0f;
         }
@@ -235,8 +235,8 @@ plan_emit_kernel(const float *__restrict__ xp, int64_t nsamp, int64_t ncl, int64
 #pragma unroll
         for (int u = 0; u < EM_U; u++) {
             line[u] = ls + 2 * (wave * EM_U + u) + half;
-            act[u] = (line[u] < l1);
             int64_t s = line[u] * LINE + sub;
+            act[u] = (line[u] < l1) && (s < nsamp);     // the last line may be partial
             y[u] = act[u] ? xp[s] : 0.0f;
             x[u] = act[u] ? xp[s + nsamp] : 0.0f;
         }
@@ -558,16 +558,21 @@ qmask_kernel(const uint64_t *__restrict__ mt, int64_t n, const float *__restrict
         if (t < total) {
             const uint64_t m = mt[e];
             zf = (q == 0) && (m & ZFLAG);
-            if (!(m & MFLAG))
+            const int64_t s0 = int64_t(uint32_t(m >> ICL_SHIFT) & ICL_MASK) * LINE + 4 * q;
+            // Groups past the end of a partial last line hold no sample.
+            if (s0 >= nsamp)
+                bit = false;
+            else if (!(m & MFLAG))
                 bit = true;
             else {
                 const uint32_t icell = uint32_t(m) & CELL_MASK;
-                const int64_t s0 = int64_t(uint32_t(m >> ICL_SHIFT) & ICL_MASK) * LINE + 4 * q;
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    Pixel<float> p;
-                    locate_pixel(xp[s0 + k], xp[nsamp + s0 + k], ny, nx, Periodic, p);
-                    bit |= (cell_key(p.iy0 >> 6, p.ix0 >> 6) == icell);
+                    if (s0 + k < nsamp) {
+                        Pixel<float> p;
+                        locate_pixel(xp[s0 + k], xp[nsamp + s0 + k], ny, nx, Periodic, p);
+                        bit |= (cell_key(p.iy0 >> 6, p.ix0 >> 6) == icell);
+                    }
                 }
             }
         }

@@ -10,7 +10,8 @@
  * errors follow the reference's wording (src_lib/check_arguments.cu:159-175).
  *
  * Array conventions (reference gpu_mm/gpu_mm.py:52-84, 117-124):
- *   tod        float[nsamp]            (a (ndet,nt) array flattened)
+ *   tod        float[nsamp]            (a (ndet,nt) array flattened); any nsamp >= 1 (the
+ *                                      reference asserts nsamp % 32 == 0, check_arguments.cu:24)
  *   xpointing  float[3][nsamp]         rows {y, x, alpha}
  *   local map  float[3*npix]           element (p,iy,ix) of cell (cy,cx) is at
  *              cell_offsets[cy,cx] + p*polstride + iy*ystride + ix
@@ -227,7 +228,7 @@ int gmm_kernel_timer_read(int which, double *total_ms, int64_t *count);
  * No counterpart in the reference (which stores TODs as float32 only). A 16-bit TOD of nsamp
  * samples in tod_nrows rows holds value / tod_scale[row] as FP16 or BF16, with one FP32 scale
  * per row; the operators compute in FP32 exactly as the float32 ones do. tod_gpu must be 8-byte
- * aligned. response_gpu may be null, or float[2][resp_ndet] as in gmm_response_map2tod. */
+ * aligned and nsamp divisible by 4. response_gpu may be null, or float[2][resp_ndet] as in gmm_response_map2tod. */
 enum { GMM_TOD_F16 = 1, GMM_TOD_BF16 = 2 };
 
 /* map2tod into a 16-bit TOD: each output is rounded once to the 16-bit type, except on lines
@@ -300,7 +301,8 @@ typedef struct {
 void gmm_batch_default_config(gmm_batch_config *cfg);
 
 /* tod_type: GMM_TOD_F32, GMM_TOD_F16 or GMM_TOD_BF16 (then tod_scale_gpu[i] is float[tod_nrows[i]],
- * otherwise both may be null). nsamp[i] is the sample count of TOD i (its plan's nsamp). */
+ * otherwise both may be null). nsamp[i] is the sample count of TOD i (its plan's nsamp), which
+ * must be divisible by 4. */
 int gmm_batch_create(gmm_batch **out, const gmm_lpix *lpix, int32_t ntod, const gmm_plan *const *plans,
                      const float *const *xpointing_gpu, void *const *tod_gpu, const int64_t *nsamp,
                      int32_t tod_type, const float *const *tod_scale_gpu, const int64_t *tod_nrows,

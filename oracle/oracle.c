@@ -389,12 +389,15 @@ void orc_pointing_params(int64_t nmt, int64_t *nmt_per_block, int64_t *nblocks)
 }
 
 /* Distinct cells (iycell << 10 | ixcell) touched by the bilinear corners of the 32 samples of
- * line 'l'; cells[0] is the cell of the (iy0, ix0) corner of the first sample. */
+ * line 'l' (fewer in a partial last line when nsamp % 32 != 0, which the reference does not
+ * allow: PointingPrePlan.cu and check_arguments.cu require nsamp % 32 == 0); cells[0] is the
+ * cell of the (iy0, ix0) corner of the first sample. */
 static int line_cells(const float *xp, int64_t nsamp, int64_t l, int ny, int nx, int periodic,
                       uint32_t cells[128], uint32_t *err)
 {
     int n = 0;
-    for (int64_t s = 32 * l; s < 32 * l + 32; s++) {
+    const int64_t s1 = (32 * l + 32 < nsamp) ? 32 * l + 32 : nsamp;
+    for (int64_t s = 32 * l; s < s1; s++) {
         int c[4];
         float d[2];
         *err |= orc_locate_f32(xp[s], xp[s + nsamp], ny, nx, periodic, c, d);
@@ -417,7 +420,7 @@ uint32_t orc_plan_count(const float *xp, int64_t nsamp, int ny, int nx, int peri
 {
     int64_t per, nb;
     orc_preplan_params(nsamp, &per, &nb);
-    int64_t ncl = nsamp / 32, total = 0;
+    int64_t ncl = (nsamp + 31) / 32, total = 0;
     uint32_t err = 0, cells[128];
     for (int64_t b = 0; b < nb; b++) {
         int64_t l1 = (b + 1) * per < ncl ? (b + 1) * per : ncl;
@@ -440,7 +443,7 @@ static int cmp_mt(const void *a, const void *b)
 uint32_t orc_plan_build(const float *xp, int64_t nsamp, int ny, int nx, int periodic,
                         uint64_t *mt, int64_t nmt)
 {
-    int64_t ncl = nsamp / 32, k = 0;
+    int64_t ncl = (nsamp + 31) / 32, k = 0;
     uint32_t err = 0, cells[128];
     for (int64_t l = 0; l < ncl; l++) {
         int n = line_cells(xp, nsamp, l, ny, nx, periodic, cells, &err);

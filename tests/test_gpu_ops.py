@@ -653,3 +653,77 @@ def test_local_map_to_global_gpu(gmm, dev):
         for p in range(3):
             v = 0.0 if o < 0 else arr[o + p * 4096 + (y % 64) * 64 + (x % 64)]
             assert got[p, y, x] == v
+
+
+# ------------------------------------------------------------------------------------------
+# TODs whose length is not a multiple of 32 (the reference requires one; here the last line of
+# a plan may be partial). nt = 4093: nsamp % 4 != 0 (scalar kernel variant, a cut last group);
+# nt = 4116: nsamp % 32 == 28 (vector variant; the last group of the last line is empty);
+# nt = 4112: nsamp % 32 == 16 (vector variant; the last four groups are empty).
+
+@pytest.mark.parametrize("nt", [4093, 4116, 4112], ids=["nsamp%4!=0", "nsamp%32==28", "nsamp%32==16"])
+@pytest.mark.parametrize("periodic", [False, True], ids=["nonperiodic", "periodic"])
+def test_partial_last_line(gmm, orc, dev, nt, periodic):
+    rng = np.random.default_rng(nt + periodic)
+    ndet, ny, nx = 3, 200, 320
+    t = np.arange(nt)
+    xp = np.empty((3, ndet, nt), F32)
+    for d in range(ndet):
+        xp[0, d] = 100 + 90 * np.sin(t / 300.0 + d)
+        xp[1, d] = (150 + 140 * np.sin(t / 170.0 + 2 * d)) if not periodic else ((t * 1.9 + 100 * d) % (2 * nx) - nx / 2)
+        xp[2, d] = rng.uniform(0, np.pi) + 0.003 * t
+    nsamp = ndet * nt
+    print(f"\n[partial last line] nsamp={nsamp} periodic={periodic}", flush=True)
+    xg = to_gpu(xp, dev)
+    pre = gmm.PointingPrePlan(xg, ny, nx, periodic_xcoord=periodic)
+    plan = gmm.PointingPlan(pre, xg)
+    cum, nmt, err = orc.plan_count(xp, ny, nx, periodic)
+    assert err == 0 and pre.plan_nmt == nmt
+    assert np.array_equal(pre.get_nmt_cumsum(), cum)
+    ref_mt, err = orc.plan_build(xp, ny, nx, periodic)
+    assert err == 0 and np.array_equal(plan.get_plan_mt(), ref_mt)
+
+    lp = gmm.LocalPixelization.make_rectangle(ny, nx, periodic_xcoord=periodic)
+    pix = orc.Pix.rectangle(ny, nx, periodic)
+    m = rng.standard_normal(3 * lp.npix).astype(F32)
+    mg = gmm.LocalMap(lp, to_gpu(m, dev))
+    ref32, _ = orc.map2tod(m, xp, pix)
+    ref64, _ = orc.map2tod(m.astype(np.float64), xp, pix, f64=True)
+    for p in (plan, None):
+        tg = torch.full((ndet, nt), 5.0, dtype=torch.float32, device=dev)
+        gmm.map2tod(tg, mg, xg, p, debug=True)
+        got = to_cpu(tg)
+        assert l1rel(got, ref32) < 1e-6 and l1rel(got, ref64) < 1e-5
+    d = rng.standard_normal((ndet, nt)).astype(F32)
+    mref32, _ = orc.tod2map(d, xp, pix)
+    mref64, _ = orc.tod2map(d.astype(np.float64), xp, pix, f64=True)
+    for p in (plan, None):
+        out = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
+        gmm.tod2map(out, to_gpu(d, dev), xg, p, debug=True)
+        assert l1rel(to_cpu(out.arr), mref32) < 1e-6 and l1rel(to_cpu(out.arr), mref64) < 1e-5
+    # Per-detector response (rows of nt samples).
+    resp = rng.uniform(0.5, 1.5, (2, ndet)).astype(F32)
+    rg = to_gpu(resp, dev)
+    rref, _ = orc.response_map2tod(m, xp, resp, pix)
+    tg = torch.empty((ndet, nt), dtype=torch.float32, device=dev)
+    gmm.map2tod(tg, mg, xg, plan, response=rg, debug=True)
+    assert l1rel(to_cpu(tg), rref) < F64_BAR          # the response oracle is FP64
+    mr, _ = orc.response_tod2map(d, xp, resp, pix)
+    out = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
+    gmm.tod2map(out, to_gpu(d, dev), xg, plan, response=rg, debug=True)
+    assert l1rel(to_cpu(out.arr), mr) < F64_BAR
+    # 16-bit TODs and batches use 4-sample vector accesses: nsamp % 4 must be 0.
+    if nsamp % 4 != 0:
+        t16, sc = gmm.encode_tod16(to_gpu(d, dev))
+        with pytest.raises(RuntimeError, match="divisible by 4"):
+            gmm.tod2map(out, t16, xg, plan, tod_scale=sc)
+        with pytest.raises(RuntimeError, match="divisible by 4"):
+            gmm.PlanBatch(lp, [plan], [xg], [tg])
+    else:
+        b = gmm.PlanBatch(lp, [plan], [xg], [tg])
+        b.map2tod(mg, debug=True)
+        assert l1rel(to_cpu(tg), ref32) < 1e-6
+        tg.copy_(to_gpu(d, dev))
+        out = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
+        b.tod2map(out, debug=True)
+        assert l1rel(to_cpu(out.arr), mref32) < 1e-6 and l1rel(to_cpu(out.arr), mref64) < 1e-5

@@ -284,10 +284,10 @@ def py_plan(xp, ny, nx, periodic):
     entries = []
     per, nb = None, None
     lines = []
-    for l in range(n // 32):
+    for l in range((n + 31) // 32):
         cells = set()
         first = None
-        for s in range(32 * l, 32 * l + 32):
+        for s in range(32 * l, min(32 * l + 32, n)):
             for (a, b) in ((iy0[s], ix0[s]), (iy0[s], ix1[s]), (iy1[s], ix0[s]), (iy1[s], ix1[s])):
                 k = ((int(a) >> 6) << 10) | (int(b) >> 6)
                 if first is None:
@@ -305,11 +305,13 @@ def py_plan(xp, ny, nx, periodic):
     return np.array(entries, dtype=np.uint64), lines, int(np.bitwise_or.reduce(err))
 
 
+# n = 32 * 700 - 13 and 32 * 700 + 4: a partial last line (nsamp % 32 != 0, which this build
+# accepts and the reference does not).
 @pytest.mark.parametrize("periodic", [False, True])
-def test_plan_matches_python_bruteforce(orc, periodic):
+@pytest.mark.parametrize("n", [32 * 700, 32 * 700 - 13, 32 * 700 + 4])
+def test_plan_matches_python_bruteforce(orc, periodic, n):
     rng = np.random.default_rng(10 + periodic)
     ny, nx = 300, 400
-    n = 32 * 700
     # Fast scan so lines often cross cells, plus wrap-around in the periodic case.
     t = np.arange(n)
     y = 150 + 140 * np.sin(t / 500.0) + rng.uniform(-1, 1, n)
