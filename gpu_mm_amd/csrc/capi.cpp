@@ -472,7 +472,7 @@ int gmm_preplan_create(gmm_preplan **out, const float *xpointing_gpu, int64_t ns
         pp.off_err = align128(8 * nmt);
         pp.off_chunks = pp.off_err + align128(4 * pp.n_err);
         pp.off_qmask = pp.off_chunks + align128(int64_t(sizeof(Chunk)) * pp.max_nchunks);
-        pp.plan_nbytes = pp.off_qmask + align128(nmt);
+        pp.plan_nbytes = pp.off_qmask + align128(2 * nmt);
 
         pp.sort_nbytes = plan_scratch_nbytes(pp) + 128;
         pp.tmp_nbytes = 2 * align128(8 * nmt) + pp.sort_nbytes;
@@ -556,7 +556,7 @@ int gmm_plan_create(gmm_plan **out, const gmm_preplan *pp, const float *xpointin
         plan->plan_mt = reinterpret_cast<uint64_t *>(plan->buf);
         plan->err = reinterpret_cast<uint32_t *>(plan->buf + pp->off_err);
         plan->chunks = reinterpret_cast<Chunk *>(plan->buf + pp->off_chunks);
-        plan->qmask = plan->buf + pp->off_qmask;
+        plan->qmask = reinterpret_cast<uint16_t *>(plan->buf + pp->off_qmask);
 
         try {
             uint8_t *tmp = reinterpret_cast<uint8_t *>(tmp_buf_gpu);

@@ -43,6 +43,17 @@ constexpr int CELL_NPIX = 64 * 64;
 // Samples per plan entry (one TOD cache line of float32).
 constexpr int LINE = 32;
 
+// Operator sample ownership (per plan entry, in the owned-quad mask qmask[e]). A sample belongs
+// to the cell of its (iy0, ix0) corner, except that the block of the cell to its left (same cell
+// row) evaluates it when the line's samples of that cell all lie in the first HALO_X - 1 columns
+// (ix0 % 64 <= HALO_X - 2, and ix0 + 1 < nx: no x-wraparound) and the line touches the left
+// cell: that block's halo tile spans HALO_X columns of its right neighbour, so a line crossing
+// the boundary is read once instead of by both blocks. Bits 0..7 of qmask[e]: the 4-sample
+// groups of the line holding a sample that entry e evaluates; QM_TAKES_RIGHT: entry e evaluates
+// the line's samples of the cell to its right.
+constexpr int HALO_X = 12;
+constexpr uint16_t QM_TAKES_RIGHT = 0x100;
+
 // Preplan block count (include/gpu_mm.hpp:63).
 constexpr int PREPLAN_SIZE = 1024;
 
@@ -239,7 +250,7 @@ struct BatchTod
     void *tod;                    // TOD [nsamp] (float, or 16-bit with per-row scales)
     const float *scale;           // 16-bit TODs: float[nsamp / scale_nper], else null
     const uint64_t *plan_mt;      // the TOD's sorted plan entries
-    const uint8_t *qmask;         // the plan's owned-quad masks
+    const uint16_t *qmask;        // the plan's owned-quad masks
     int64_t nsamp;
     uint32_t scale_nper;          // samples per scale row
     uint32_t pad;

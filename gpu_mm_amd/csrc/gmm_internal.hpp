@@ -53,7 +53,7 @@ struct gmm_plan
     uint64_t *plan_mt = nullptr;    // sorted plan entries
     uint32_t *err = nullptr;        // per-block error words (n_err of them)
     gmm::Chunk *chunks = nullptr;   // operator work list
-    uint8_t *qmask = nullptr;       // per entry: the quads of its line whose samples it owns
+    uint16_t *qmask = nullptr;      // per entry: the quads of its line it evaluates (+ QM_TAKES_RIGHT)
     int64_t nchunks = 0;
     int64_t nmlines = 0;
 };
@@ -135,7 +135,7 @@ void plan_sort(const gmm_preplan &pp, uint64_t *a, uint64_t *b, uint64_t *out,
 // Builds the chunk list and the multi-cell line list from the sorted plan. Writes the
 // two counts to counts_gpu[0], counts_gpu[1].
 void plan_schedule(const gmm_preplan &pp, const uint64_t *sorted_mt, const float *xp, Chunk *chunks,
-                   uint8_t *qmask, uint32_t *counts_gpu, void *scratch,
+                   uint16_t *qmask, uint32_t *counts_gpu, void *scratch,
                    int64_t scratch_nbytes, hipStream_t stream);
 
 // Size of the scratch used by plan_sort() + plan_schedule().

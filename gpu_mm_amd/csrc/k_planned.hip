@@ -7,12 +7,16 @@
 //
 // Ownership. A sample belongs to the cell of its (iy0, ix0) corner; that cell is in the plan
 // for the sample's line, since the plan lists every corner's cell. The block of a cell works on
-// a 65x65 "halo" tile per plane: the cell plus the first row of the cell below and the first
-// column of the cell to the right (for periodic x, the column that wraparound reads), so it
-// evaluates or deposits all four corners of each sample it owns at tile offsets 0, 1, 65, 66,
-// and skips the samples it does not own. Lines that touch several cells are thus written once
-// per sample by plain stores: no zeroing pass and no atomics on the TOD (the reference zeroes
-// such lines and accumulates them atomically, src_lib/map2tod.cu:15-62, 81-191).
+// a 65 x (64 + HALO_X) "halo" tile per plane: the cell plus the first row of the cells below and
+// the first HALO_X columns of the cell to the right (for periodic x in the last cell column, the
+// column that wraparound reads), so it evaluates or deposits all four corners of each sample it
+// evaluates at tile offsets o, o+1, o+HS, o+HS+1, and skips the others. It evaluates the samples
+// it owns, and also those of the right-hand cell on lines where the plan's owned-quad mask says
+// so (QM_TAKES_RIGHT: the line's part in that cell lies within the halo columns), while that
+// cell's entry for the line then loads nothing: a line that crosses a vertical cell boundary is
+// read once. Lines that touch several cells are written once per sample by plain stores: no
+// zeroing pass and no atomics on the TOD (the reference zeroes such lines and accumulates them
+// atomically, src_lib/map2tod.cu:15-62, 81-191).
 //
 // tod2map accumulates in LDS with 64-bit integer atomics on fixed-point values: float LDS
 // atomics (ds_add_f32) run an order of magnitude slower than integer ones on this GPU. Deposits
@@ -21,8 +25,8 @@
 // which is more accurate than float32 accumulation. The bound is estimated in the kernel (4x
 // the largest |t| in the first lines of the chunk); a chunk holding a larger value runs again
 // with its exact maximum, and one holding Inf/NaN runs again with float LDS atomics. The halo
-// row and column, and the cell's own first row and column (other blocks' halo targets), are
-// flushed to the map with atomics, the rest with a plain read-modify-write.
+// row and columns, and the cell's own first row and first HALO_X columns (other blocks' halo
+// targets), are flushed to the map with atomics, the rest with a plain read-modify-write.
 // Reference: src_lib/tod2map.cu:33-154.
 //
 // Both kernels also take per-detector gains, float response[2][ndet] (row 0 scales T, row 1
@@ -36,8 +40,9 @@ namespace gmm {
 namespace {
 
 constexpr int WAVE = 64;
-constexpr int HS = 65;              // halo tile side: the cell plus one row and column of neighbours
-constexpr int HPLANE = HS * HS;     // floats per halo tile plane
+constexpr int HROWS = 65;           // halo tile rows: the cell plus row 0 of the cells below
+constexpr int HS = 64 + HALO_X;     // halo tile row stride: the cell plus HALO_X columns to the right
+constexpr int HPLANE = HROWS * HS;  // pixels per halo tile plane
 
 constexpr int M2T_THREADS = 512;
 constexpr int T2M_THREADS = 1024;
@@ -361,16 +366,21 @@ __device__ inline void locate_op(float y, float x, const Geom &g, Checks &c, int
 }
 
 // ------------------------------------------------------------------------------------------
-// Halo tile: the block's cell (64x64 per plane) plus row 64 and column W from the neighbouring
-// cells below and to the right, where W = 64, or, in periodic mode for the last cell column when
-// nx is not a multiple of 64, W = (nx-1) % 64 + 1: the column that x-wraparound (ix1 = 0) reads.
-// Absent cells read as zeros. With this tile, the block that owns a sample (the cell of its
-// (iy0, ix0) corner) evaluates all four corners at tile offsets 0, 1, HS, HS+1.
+// Halo tile (HROWS x HS per plane): the block's cell (64x64) plus row 64 from the cells below and
+// columns W .. W+ncol-1 from the cells to the right (and below right). Normally W = 64 and ncol =
+// HALO_X: the right-hand cell's first HALO_X columns, so the block can evaluate the samples of
+// that cell which the ownership rule gives it (local column < 64 + HALO_X - 1). For the last
+// cell column ncol = 1: in periodic mode W = (nx-1) % 64 + 1 holds the column that x-wraparound
+// (ix1 = 0) reads (W = 64 when nx is a multiple of 64), otherwise there is no right-hand cell.
+// Absent cells read as zeros. A sample at local (ly, lx) has its four corners at tile offsets
+// o, o+1, o+HS, o+HS+1 with o = ly*HS + lx.
 
 struct HaloCells
 {
     int64_t own, down, right, diag;   // cell offsets (-1 = absent)
-    int w;                            // halo column
+    int w;                            // first tile column taken from the right-hand cell
+    int ncol;                         // number of such columns
+    int cx;                           // the block's cell column
 };
 
 __device__ inline int64_t cell_off(const int64_t *__restrict__ cell_offsets, int cy, int cx, int nycells, int nxcells)
@@ -391,6 +401,8 @@ __device__ inline HaloCells halo_cells(uint32_t icell, const int64_t *__restrict
     h.right = cell_off(cell_offsets, cy, rcx, nycells, nxcells);
     h.diag = cell_off(cell_offsets, cy + 1, rcx, nycells, nxcells);
     h.w = wrap ? ((nx - 1) & 63) + 1 : 64;
+    h.ncol = (cx < last_cx) ? HALO_X : 1;
+    h.cx = cx;
     return h;
 }
 
@@ -435,19 +447,22 @@ __device__ inline void stage_halo_tile(float4 *__restrict__ tile, const float *_
         if (c0 + 2 != h.w) row[2] = make_float4(t[k].z, q[k].z, u[k].z, 0.0f);
         if (c0 + 3 != h.w) row[3] = make_float4(t[k].w, q[k].w, u[k].w, 0.0f);
     }
-    // Row 64 (columns 0..W) and column W (rows 0..63).
-    const int nrow = h.w + 1, nper = nrow + 64;
+    // Row 64 (columns 0 .. W+ncol-1) and columns W .. W+ncol-1 of rows 0..63.
+    const int nrow = h.w + h.ncol, nper = nrow + 64 * h.ncol;
     for (int j = tid; j < nper; j += NT) {
         int64_t src;
         int idx;
         if (j < nrow) {
             idx = 64 * HS + j;
-            src = (j < h.w) ? ((h.down >= 0) ? h.down + j : -1) : h.diag;
+            if (j < h.w)
+                src = (h.down >= 0) ? h.down + j : -1;
+            else
+                src = (h.diag >= 0) ? h.diag + (j - h.w) : -1;
         }
         else {
-            const int r = j - nrow;
-            idx = r * HS + h.w;
-            src = (h.right >= 0) ? h.right + r * ystride : -1;
+            const int r = (j - nrow) / h.ncol, c = (j - nrow) - r * h.ncol;
+            idx = r * HS + h.w + c;
+            src = (h.right >= 0) ? h.right + r * ystride + c : -1;
         }
         float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (src >= 0) {
@@ -536,14 +551,14 @@ __device__ inline bool quad_small_angles(const float4 &a4)
 
 template<bool Periodic, bool HasResp, bool Small>
 __device__ inline float m2t_sample(const float4 *__restrict__ tile, const float2 *__restrict__ tab, float y, float x,
-                                   float alpha, const Geom &g, int ybase, int xbase, float tr, float pr, Checks &chk,
-                                   bool &owned)
+                                   float alpha, const Geom &g, int ybase, int xbase, unsigned lxlim, float tr, float pr,
+                                   Checks &chk, bool &owned)
 {
     int iy0, ix0;
     float dy, dx;
     locate_op<Periodic>(y, x, g, chk, iy0, ix0, dy, dx);
     const int ly = iy0 - ybase, lx = ix0 - xbase;
-    owned = (unsigned(ly) < 64u) && (unsigned(lx) < 64u);
+    owned = (unsigned(ly) < 64u) && (unsigned(lx) < lxlim);
     float sn, cs;
     sincos2<Small>(tab, alpha, sn, cs);
     const int o = owned ? (ly * HS + lx) : 0;
@@ -555,7 +570,7 @@ __device__ inline float m2t_sample(const float4 *__restrict__ tile, const float2
 // the block that owns it: no atomics and no zeroing.
 template<bool Periodic, bool Vec, bool HasResp, typename TT>
 __device__ inline void m2t_entries(TT *__restrict__ tod, const float *__restrict__ xp,
-                                   const uint64_t *__restrict__ plan_mt, const uint8_t *__restrict__ qmask,
+                                   const uint64_t *__restrict__ plan_mt, const uint16_t *__restrict__ qmask,
                                    int64_t nsamp, uint32_t wb, uint32_t we, const float4 *__restrict__ tile,
                                    const float2 *__restrict__ sct, const Geom &g, int ybase, int xbase, bool accum,
                                    const Gains &gn, Checks &chk, uint32_t &err)
@@ -574,11 +589,13 @@ __device__ inline void m2t_entries(TT *__restrict__ tod, const float *__restrict
             const uint32_t idx = j + li;
             const uint32_t e = __shfl(packed, int(min(idx, 63u)), WAVE);
             const uint32_t qm = __shfl(qmw, int(min(idx, 63u)), WAVE);
+            // Columns of the right-hand cell that this entry evaluates (ownership rule at HALO_X).
+            const unsigned lxlim = (qm & QM_TAKES_RIGHT) ? unsigned(64 + HALO_X - 1) : 64u;
             const uint32_t icl = e & ICL_MASK;
             const bool act = (idx < cnt) && (icl < ncl);
             if (idx < cnt && icl >= ncl)
                 err |= ERR_INCONSISTENT_NMT;
-            // Quads without a sample owned by this cell are neither loaded nor evaluated.
+            // Quads without a sample this entry evaluates are neither loaded nor evaluated.
             if (!act || !((qm >> q) & 1u))
                 continue;
             const int64_t s0 = int64_t(icl) * LINE + 4 * q;
@@ -595,19 +612,19 @@ __device__ inline void m2t_entries(TT *__restrict__ tod, const float *__restrict
             if (quad_small_angles(a4)) {
                 // Every angle of the wave's groups takes the table path: straight-line code, so
                 // the LDS reads of the four samples overlap.
-                v.x = m2t_sample<Periodic, HasResp, true>(tile, sct, y4.x, x4.x, a4.x, g, ybase, xbase, tr[0], pr[0], chk, o0);
-                v.y = m2t_sample<Periodic, HasResp, true>(tile, sct, y4.y, x4.y, a4.y, g, ybase, xbase, tr[1], pr[1], chk, o1);
-                v.z = m2t_sample<Periodic, HasResp, true>(tile, sct, y4.z, x4.z, a4.z, g, ybase, xbase, tr[2], pr[2], chk, o2);
-                v.w = m2t_sample<Periodic, HasResp, true>(tile, sct, y4.w, x4.w, a4.w, g, ybase, xbase, tr[3], pr[3], chk, o3);
+                v.x = m2t_sample<Periodic, HasResp, true>(tile, sct, y4.x, x4.x, a4.x, g, ybase, xbase, lxlim, tr[0], pr[0], chk, o0);
+                v.y = m2t_sample<Periodic, HasResp, true>(tile, sct, y4.y, x4.y, a4.y, g, ybase, xbase, lxlim, tr[1], pr[1], chk, o1);
+                v.z = m2t_sample<Periodic, HasResp, true>(tile, sct, y4.z, x4.z, a4.z, g, ybase, xbase, lxlim, tr[2], pr[2], chk, o2);
+                v.w = m2t_sample<Periodic, HasResp, true>(tile, sct, y4.w, x4.w, a4.w, g, ybase, xbase, lxlim, tr[3], pr[3], chk, o3);
             }
             else {
-                v.x = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.x, x4.x, a4.x, g, ybase, xbase, tr[0], pr[0], chk, o0);
+                v.x = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.x, x4.x, a4.x, g, ybase, xbase, lxlim, tr[0], pr[0], chk, o0);
                 __builtin_amdgcn_sched_barrier(0);
-                v.y = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.y, x4.y, a4.y, g, ybase, xbase, tr[1], pr[1], chk, o1);
+                v.y = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.y, x4.y, a4.y, g, ybase, xbase, lxlim, tr[1], pr[1], chk, o1);
                 __builtin_amdgcn_sched_barrier(0);
-                v.z = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.z, x4.z, a4.z, g, ybase, xbase, tr[2], pr[2], chk, o2);
+                v.z = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.z, x4.z, a4.z, g, ybase, xbase, lxlim, tr[2], pr[2], chk, o2);
                 __builtin_amdgcn_sched_barrier(0);
-                v.w = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.w, x4.w, a4.w, g, ybase, xbase, tr[3], pr[3], chk, o3);
+                v.w = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.w, x4.w, a4.w, g, ybase, xbase, lxlim, tr[3], pr[3], chk, o3);
             }
             // Samples past the end of the TOD (read as zeros) are not written.
             o1 = o1 && nv > 1;
@@ -643,11 +660,11 @@ template<bool Periodic, bool Vec, bool HasResp, typename TT>
 __global__ void __launch_bounds__(M2T_THREADS, 4)
 planned_map2tod_kernel(TT *__restrict__ tod, const float *__restrict__ lmap, const float *__restrict__ xp,
                        const int64_t *__restrict__ cell_offsets, const uint64_t *__restrict__ plan_mt,
-                       const Chunk *__restrict__ chunks, const uint8_t *__restrict__ qmask,
+                       const Chunk *__restrict__ chunks, const uint16_t *__restrict__ qmask,
                        uint32_t *__restrict__ err_word, int64_t nsamp, int ny, int nx, int nycells, int nxcells,
                        int64_t ystride, int64_t polstride, bool partial, bool accum, Gains gn)
 {
-    __shared__ float4 tile[HPLANE];                       // 66 KiB
+    __shared__ float4 tile[HPLANE];                       // 77 KiB
     __shared__ float2 sct[SCT];
 
     const Chunk ch = chunks[xcd_chunk(blockIdx.x, gridDim.x)];
@@ -703,18 +720,19 @@ __device__ inline void deposit4(unsigned long long *__restrict__ acc, float *__r
     }
 }
 
-// One sample of tod2map: locate; if the block owns it (its (iy0, ix0) corner lies in the
-// block's cell), deposit all four corners into the halo tile.
+// One sample of tod2map: locate; if the block evaluates it (its (iy0, ix0) corner lies in the
+// block's cell, or at local column < lxlim of the cell to the right), deposit all four corners
+// into the halo tile.
 template<bool Periodic, bool HasResp, bool Fixed, bool Small>
 __device__ inline void t2m_sample(unsigned long long *__restrict__ acc, float *__restrict__ facc,
                                   const float2 *__restrict__ tab, float y, float x, float alpha, float t, const Geom &g,
-                                  int ybase, int xbase, float tr, float pr, Checks &chk)
+                                  int ybase, int xbase, unsigned lxlim, float tr, float pr, Checks &chk)
 {
     int iy0, ix0;
     float dy, dx;
     locate_op<Periodic>(y, x, g, chk, iy0, ix0, dy, dx);
     const int ly = iy0 - ybase, lx = ix0 - xbase;
-    if ((unsigned(ly) < 64u) && (unsigned(lx) < 64u)) {
+    if ((unsigned(ly) < 64u) && (unsigned(lx) < lxlim)) {
         float sn, cs;
         sincos2<Small>(tab, alpha, sn, cs);
         const float ts = HasResp ? t * tr : t;
@@ -764,7 +782,7 @@ __device__ inline float t2m_probe(const TT *__restrict__ tod, const uint64_t *__
 template<bool Periodic, bool Vec, bool HasResp, typename TT>
 __device__ inline void t2m_entries(unsigned long long *__restrict__ acc, float *__restrict__ facc,
                                    const TT *__restrict__ tod, const float *__restrict__ xp,
-                                   const uint64_t *__restrict__ plan_mt, const uint8_t *__restrict__ qmask,
+                                   const uint64_t *__restrict__ plan_mt, const uint16_t *__restrict__ qmask,
                                    int64_t nsamp, uint32_t wb, uint32_t we, const float2 *__restrict__ sct,
                                    const Geom &g, int ybase, int xbase, const Gains &gn, bool finite, float scale,
                                    float &mrun, float &tsum, Checks &chk, uint32_t &err)
@@ -784,6 +802,8 @@ __device__ inline void t2m_entries(unsigned long long *__restrict__ acc, float *
             const uint32_t idx = j + li;
             const uint32_t e2 = __shfl(packed, int(min(idx, 63u)), WAVE);
             const uint32_t qm = __shfl(qmw, int(min(idx, 63u)), WAVE);
+            // Columns of the right-hand cell that this entry evaluates (ownership rule at HALO_X).
+            const unsigned lxlim = (qm & QM_TAKES_RIGHT) ? unsigned(64 + HALO_X - 1) : 64u;
             const uint32_t icl = e2 & ICL_MASK;
             const bool act = (idx < cnt) && (icl < ncl);
             if (idx < cnt && icl >= ncl)
@@ -822,23 +842,23 @@ __device__ inline void t2m_entries(unsigned long long *__restrict__ acc, float *
                 t4.z *= scale;
                 t4.w *= scale;
                 if (quad_small_angles(a4)) {
-                    t2m_sample<Periodic, HasResp, true, true>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, tr[0], pr[0], chk);
-                    t2m_sample<Periodic, HasResp, true, true>(acc, facc, sct, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, tr[1], pr[1], chk);
-                    t2m_sample<Periodic, HasResp, true, true>(acc, facc, sct, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, tr[2], pr[2], chk);
-                    t2m_sample<Periodic, HasResp, true, true>(acc, facc, sct, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, tr[3], pr[3], chk);
+                    t2m_sample<Periodic, HasResp, true, true>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, lxlim, tr[0], pr[0], chk);
+                    t2m_sample<Periodic, HasResp, true, true>(acc, facc, sct, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, lxlim, tr[1], pr[1], chk);
+                    t2m_sample<Periodic, HasResp, true, true>(acc, facc, sct, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, lxlim, tr[2], pr[2], chk);
+                    t2m_sample<Periodic, HasResp, true, true>(acc, facc, sct, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, lxlim, tr[3], pr[3], chk);
                 }
                 else {
-                    t2m_sample<Periodic, HasResp, true, false>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, tr[0], pr[0], chk);
-                    t2m_sample<Periodic, HasResp, true, false>(acc, facc, sct, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, tr[1], pr[1], chk);
-                    t2m_sample<Periodic, HasResp, true, false>(acc, facc, sct, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, tr[2], pr[2], chk);
-                    t2m_sample<Periodic, HasResp, true, false>(acc, facc, sct, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, tr[3], pr[3], chk);
+                    t2m_sample<Periodic, HasResp, true, false>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, lxlim, tr[0], pr[0], chk);
+                    t2m_sample<Periodic, HasResp, true, false>(acc, facc, sct, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, lxlim, tr[1], pr[1], chk);
+                    t2m_sample<Periodic, HasResp, true, false>(acc, facc, sct, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, lxlim, tr[2], pr[2], chk);
+                    t2m_sample<Periodic, HasResp, true, false>(acc, facc, sct, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, lxlim, tr[3], pr[3], chk);
                 }
             }
             else {
-                t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, tr[0], pr[0], chk);
-                t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, tr[1], pr[1], chk);
-                t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, tr[2], pr[2], chk);
-                t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, tr[3], pr[3], chk);
+                t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, lxlim, tr[0], pr[0], chk);
+                t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, lxlim, tr[1], pr[1], chk);
+                t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, lxlim, tr[2], pr[2], chk);
+                t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, lxlim, tr[3], pr[3], chk);
             }
         }
     }
@@ -855,13 +875,11 @@ __device__ inline int t2m_scale_exp(uint32_t mbits)
 template<int NT>
 __device__ inline void t2m_clear(unsigned long long *__restrict__ acc, bool finite, int tid)
 {
-    constexpr int NPAIR = (3 * HPLANE + 1) / 2;
+    static_assert((3 * HPLANE) % 4 == 0, "the float view's half of the tile is a whole number of 16-byte stores");
     ulonglong2 *a2 = reinterpret_cast<ulonglong2 *>(acc);
-    const int n = finite ? (3 * HPLANE) / 2 : (NPAIR + 1) / 2;
+    const int n = finite ? (3 * HPLANE) / 2 : (3 * HPLANE) / 4;
     for (int i = tid; i < n; i += NT)
         a2[i] = make_ulonglong2(0ull, 0ull);
-    if (finite && tid == 0)
-        acc[3 * HPLANE - 1] = 0ull;               // 3 * HPLANE is odd
 }
 
 __device__ inline float t2m_value(const unsigned long long *__restrict__ acc, const float *__restrict__ facc, int k,
@@ -870,19 +888,21 @@ __device__ inline float t2m_value(const unsigned long long *__restrict__ acc, co
     return finite ? float((long long)acc[k]) * inv_scale : facc[k];
 }
 
-// Flush of the halo tile into the map. The cell's row 0 and column 0 also receive other
-// blocks' halo deposits (and column 0 of cell column 0 the wrapped ones), so they and every
-// split cell are added atomically; the rest of the cell with a plain read-modify-write. A split
-// cell is flushed one pixel per lane (each atomic instruction then touches few cache lines); an
-// unsplit one as float4 groups of 4 pixels per thread where the group allows it.
-// Halo row 64 and column W go to the neighbouring cells, atomically; deposits into absent cells
-// are dropped.
+// Flush of the halo tile into the map. The cell's row 0 and its first HALO_X columns also
+// receive other blocks' halo deposits (column 0 of cell column 0 the wrapped ones), so they and
+// every split cell are added atomically; the rest of the cell with a plain read-modify-write. A
+// split cell is flushed one pixel per lane (each atomic instruction then touches few cache
+// lines); an unsplit one as float4 groups of 4 pixels per thread where the group allows it.
+// Halo row 64 and columns W .. W+ncol-1 go to the neighbouring cells, atomically; deposits into
+// absent cells are dropped.
 template<int NT>
 __device__ inline void t2m_flush(float *__restrict__ lmap, const unsigned long long *__restrict__ acc,
                                  const float *__restrict__ facc, const HaloCells &h, bool shared, bool finite,
                                  float inv_scale, int64_t ystride, int64_t polstride, int tid)
 {
     const bool vec = !shared && ((reinterpret_cast<uintptr_t>(lmap) & 15) == 0);
+    // Columns [0, catom) may receive the left-hand cell's halo deposits.
+    const int catom = (h.cx >= 1) ? HALO_X : 1;
     if (h.own >= 0 && !vec) {
         // Whole cell, one pixel per lane; atomic where another block may add too.
         for (int i = tid; i < 3 * CELL_NPIX; i += NT) {
@@ -891,16 +911,16 @@ __device__ inline void t2m_flush(float *__restrict__ lmap, const unsigned long l
                 continue;
             const float v = t2m_value(acc, facc, pl * HPLANE + r * HS + c, finite, inv_scale);
             float *gp = lmap + h.own + pl * polstride + r * ystride + c;
-            if (shared || r == 0 || c == 0)
+            if (shared || r == 0 || c < catom)
                 unsafeAtomicAdd(gp, v);
             else
                 *gp += v;
         }
     }
     else if (h.own >= 0) {
-        // Groups of 4 pixels of a row, all planes; a group holding row 0, column 0 or column W
-        // is done pixel by pixel in its thread, the others as float4 read-modify-writes. All of
-        // a thread's float4 loads are issued first.
+        // Groups of 4 pixels of a row, all planes; a group holding row 0, a column below catom
+        // or column W is done pixel by pixel in its thread, the others as float4
+        // read-modify-writes. All of a thread's float4 loads are issued first.
         constexpr int NGRP = 3 * CELL_NPIX / 4;
         constexpr int NG = (NGRP + NT - 1) / NT;
         float4 old[NG];
@@ -909,7 +929,7 @@ __device__ inline void t2m_flush(float *__restrict__ lmap, const unsigned long l
         for (int k = 0; k < NG; k++) {
             const int g = tid + k * NT;
             const int pl = g >> 10, r = (g >> 4) & 63, c0 = (g & 15) << 2;
-            full[k] = (g < NGRP) && r != 0 && c0 != 0 && (h.w < c0 || h.w > c0 + 3);
+            full[k] = (g < NGRP) && r != 0 && c0 >= catom && (h.w < c0 || h.w > c0 + 3);
             if (full[k])
                 old[k] = *reinterpret_cast<const float4 *>(lmap + h.own + pl * polstride + r * ystride + c0);
         }
@@ -936,7 +956,7 @@ __device__ inline void t2m_flush(float *__restrict__ lmap, const unsigned long l
                     if (c == h.w)
                         continue;
                     const float v = t2m_value(acc, facc, kt + j, finite, inv_scale);
-                    if (r == 0 || c == 0)
+                    if (r == 0 || c < catom)
                         unsafeAtomicAdd(gp + j, v);
                     else
                         gp[j] += v;
@@ -944,8 +964,9 @@ __device__ inline void t2m_flush(float *__restrict__ lmap, const unsigned long l
             }
         }
     }
-    // Halo row 64 (columns 0..W) and column W (rows 0..63), to the neighbouring cells.
-    const int nrow = h.w + 1, nper = nrow + 64;
+    // Halo row 64 (columns 0 .. W+ncol-1) and columns W .. W+ncol-1 of rows 0..63, to the
+    // neighbouring cells.
+    const int nrow = h.w + h.ncol, nper = nrow + 64 * h.ncol;
     for (int i = tid; i < 3 * nper; i += NT) {
         const int pl = i / nper;
         const int jj = i - pl * nper;
@@ -953,12 +974,15 @@ __device__ inline void t2m_flush(float *__restrict__ lmap, const unsigned long l
         int64_t dst;
         if (jj < nrow) {
             k = 64 * HS + jj;
-            dst = (jj < h.w) ? ((h.down >= 0) ? h.down + jj : -1) : h.diag;
+            if (jj < h.w)
+                dst = (h.down >= 0) ? h.down + jj : -1;
+            else
+                dst = (h.diag >= 0) ? h.diag + (jj - h.w) : -1;
         }
         else {
-            const int r = jj - nrow;
-            k = r * HS + h.w;
-            dst = (h.right >= 0) ? h.right + r * ystride : -1;
+            const int r = (jj - nrow) / h.ncol, c = (jj - nrow) - r * h.ncol;
+            k = r * HS + h.w + c;
+            dst = (h.right >= 0) ? h.right + r * ystride + c : -1;
         }
         if (dst < 0)
             continue;
@@ -1022,11 +1046,11 @@ template<bool Periodic, bool Vec, bool HasResp, typename TT>
 __global__ void __launch_bounds__(T2M_THREADS)
 planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, const float *__restrict__ xp,
                        const int64_t *__restrict__ cell_offsets, const uint64_t *__restrict__ plan_mt,
-                       const Chunk *__restrict__ chunks, const uint8_t *__restrict__ qmask,
+                       const Chunk *__restrict__ chunks, const uint16_t *__restrict__ qmask,
                        uint32_t *__restrict__ err_word, int64_t nsamp, int ny, int nx, int nycells, int nxcells,
                        int64_t ystride, int64_t polstride, bool partial, Gains gn)
 {
-    __shared__ __align__(16) unsigned long long acc[3 * HPLANE];       // 99 KiB
+    __shared__ __align__(16) unsigned long long acc[3 * HPLANE];       // 116 KiB
     __shared__ float2 sct[SCT];
     __shared__ uint32_t s_max, s_redo[3];
     float *facc = reinterpret_cast<float *>(acc);         // float tile for the non-finite fallback
@@ -1164,7 +1188,7 @@ batched_map2tod_kernel(const float *__restrict__ lmap, const BatchTod *__restric
                        const BatchSeg *__restrict__ segs, const BatchItem *__restrict__ items,
                        const int64_t *__restrict__ cell_offsets, BatchArgs ba)
 {
-    __shared__ float4 tile[HPLANE];                       // 66 KiB
+    __shared__ float4 tile[HPLANE];                       // 77 KiB
     __shared__ float2 sct[SCT];
     __shared__ uint32_t s_item;
 
@@ -1208,7 +1232,7 @@ batched_tod2map_kernel(float *__restrict__ lmap, const BatchTod *__restrict__ to
                        const BatchSeg *__restrict__ segs, const BatchItem *__restrict__ items,
                        const int64_t *__restrict__ cell_offsets, BatchArgs ba)
 {
-    __shared__ __align__(16) unsigned long long acc[3 * HPLANE];       // 99 KiB
+    __shared__ __align__(16) unsigned long long acc[3 * HPLANE];       // 116 KiB
     __shared__ float2 sct[SCT];
     __shared__ uint32_t s_max, s_redo[3], s_item;
     float *facc = reinterpret_cast<float *>(acc);

@@ -535,50 +535,104 @@ __global__ void chunk_fill_kernel(const uint32_t *__restrict__ begin, const uint
     }
 }
 
-// Owned-quad masks for the operators: bit q of qmask[e] is set iff some sample of quad q
-// (samples 4q..4q+3) of entry e's line has its (iy0, ix0) corner in entry e's cell. A line that
-// lies in one cell owns all 8 quads. Also counts the multi-cell lines (one zflag entry each).
-// Eight consecutive threads handle the eight quads of one entry.
+// Owned-quad masks for the operators (ownership rule at HALO_X in gmm_common.hpp). For entry e
+// of cell Y = (cy, cx) and line l, with has(X) = "l has a sample whose (iy0, ix0) corner is in
+// X", fit(X) = "every such sample has ix0 % 64 <= HALO_X - 2 and ix0 + 1 < nx" and touches(X) =
+// "a bilinear corner of some sample of l is in X" (exactly: the plan has an entry (l, X)):
+//   eligible(X) = has(X) && fit(X) && touches(X - 1)        (X - 1: the cell to the left)
+//   R(l, X)     = has(X + 1) && fit(X + 1) && !eligible(X)  (X's entry evaluates X + 1's samples)
+// A sample of cell X is evaluated by X - 1 iff R(l, X - 1), else by X. R(l, X) excludes that
+// X's own samples go left, so no block both gives and takes, and each sample has exactly one
+// evaluator; every entry of l computes these from the same 32 samples, so they agree. Bit q of
+// qmask[e] is set iff quad q (samples 4q..4q+3) holds a sample that Y evaluates; QM_TAKES_RIGHT
+// is R(l, Y). A line that lies in one cell gives all 8 quads to it. Also counts the multi-cell
+// lines (one zflag entry each). Eight consecutive threads handle the eight quads of one entry.
 template<bool Periodic>
 __global__ void __launch_bounds__(256)
 qmask_kernel(const uint64_t *__restrict__ mt, int64_t n, const float *__restrict__ xp, int64_t nsamp, int ny,
-             int nx, uint8_t *__restrict__ qmask, uint32_t *__restrict__ nmlines)
+             int nx, uint16_t *__restrict__ qmask, uint32_t *__restrict__ nmlines)
 {
     __shared__ uint32_t s_nml;
     if (threadIdx.x == 0)
         s_nml = 0;
     __syncthreads();
     const int lane = lane_id();
+    const int nxcells = (nx + 63) >> 6;
     const int64_t total = 8 * n;
+    // Line summary bits, OR-reduced over the entry's 32 samples.
+    constexpr uint32_t HAS_L = 1, BAD_L = 2, HAS_Y = 4, BAD_Y = 8, HAS_R = 16, BAD_R = 32, TOUCH_L = 64,
+                       TOUCH_LL = 128;
     for (int64_t t0 = int64_t(blockIdx.x) * 256; t0 < total; t0 += int64_t(gridDim.x) * 256) {
         const int64_t t = t0 + threadIdx.x;
         const int64_t e = t >> 3;
         const int q = int(t & 7);
-        bool bit = false, zf = false;
+        bool zf = false, multi = false;
+        uint32_t icell = 0, sum = 0, own_y = 0, own_r = 0;     // own_*: per sample k, bit k
         if (t < total) {
             const uint64_t m = mt[e];
             zf = (q == 0) && (m & ZFLAG);
+            multi = (m & MFLAG) != 0;
+            icell = uint32_t(m) & CELL_MASK;
+            const int cy = int(icell >> 10), cx = int(icell & 1023u);
+            const uint32_t kl = (cx >= 1) ? cell_key(cy, cx - 1) : NO_CELL;
+            const uint32_t kll = (cx >= 2) ? cell_key(cy, cx - 2) : NO_CELL;
+            const uint32_t kr = (cx + 1 < nxcells) ? cell_key(cy, cx + 1) : NO_CELL;
             const int64_t s0 = int64_t(uint32_t(m >> ICL_SHIFT) & ICL_MASK) * LINE + 4 * q;
-            // Groups past the end of a partial last line hold no sample.
-            if (s0 >= nsamp)
-                bit = false;
-            else if (!(m & MFLAG))
-                bit = true;
-            else {
-                const uint32_t icell = uint32_t(m) & CELL_MASK;
+            // Samples past the end of a partial last line do not exist.
+            const uint32_t nv = uint32_t(max<int64_t>(0, min<int64_t>(4, nsamp - s0)));
+            if (!multi)
+                own_y = (1u << nv) - 1u;
+            float yv[4], xv[4];
+            if (multi && nv > 0) {
+                // All loads first (indices clamped to the last sample), so that they overlap.
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    if (s0 + k < nsamp) {
-                        Pixel<float> p;
-                        locate_pixel(xp[s0 + k], xp[nsamp + s0 + k], ny, nx, Periodic, p);
-                        bit |= (cell_key(p.iy0 >> 6, p.ix0 >> 6) == icell);
-                    }
+                    const int64_t sk = s0 + min<int64_t>(k, nv - 1);
+                    yv[k] = xp[sk];
+                    xv[k] = xp[nsamp + sk];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (!multi || uint32_t(k) >= nv)
+                    continue;
+                Pixel<float> p;
+                locate_pixel(yv[k], xv[k], ny, nx, Periodic, p);
+                uint32_t fk[4];
+                footprint_cells(p, fk);
+                const uint32_t o = fk[0];
+                const bool fit = ((p.ix0 & 63) <= HALO_X - 2) && (p.ix0 + 1 < nx);
+                if (o == kl)
+                    sum |= fit ? HAS_L : (HAS_L | BAD_L);
+                if (o == icell) {
+                    sum |= fit ? HAS_Y : (HAS_Y | BAD_Y);
+                    own_y |= 1u << k;
+                }
+                if (o == kr) {
+                    sum |= fit ? HAS_R : (HAS_R | BAD_R);
+                    own_r |= 1u << k;
+                }
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    if (fk[c] != NO_CELL && fk[c] == kl)
+                        sum |= TOUCH_L;
+                    if (fk[c] != NO_CELL && fk[c] == kll)
+                        sum |= TOUCH_LL;
                 }
             }
         }
+        // OR over the 8 lanes of the entry (aligned groups of 8 within the wave).
+        sum |= __shfl_xor(sum, 1, WAVE);
+        sum |= __shfl_xor(sum, 2, WAVE);
+        sum |= __shfl_xor(sum, 4, WAVE);
+        const bool elig_l = (sum & HAS_L) && !(sum & BAD_L) && (sum & TOUCH_LL);
+        const bool elig_y = (sum & HAS_Y) && !(sum & BAD_Y) && (sum & TOUCH_L);
+        const bool taken_y = elig_y && !elig_l;                  // R(l, Y - 1)
+        const bool takes_r = (sum & HAS_R) && !(sum & BAD_R) && !elig_y;   // R(l, Y)
+        const bool bit = (own_y != 0 && !taken_y) || (own_r != 0 && takes_r);
         const uint64_t b = __ballot(bit);
         if (t < total && q == 0)
-            qmask[e] = uint8_t((b >> (lane & ~7)) & 0xffu);
+            qmask[e] = uint16_t(((b >> (lane & ~7)) & 0xffu) | (takes_r ? QM_TAKES_RIGHT : 0u));
         const uint64_t z = __ballot(zf);
         if (lane == 0 && z)
             atomicAdd(&s_nml, uint32_t(__popcll(z)));
@@ -817,7 +871,7 @@ void build_chunks(const uint64_t *mt, int64_t n, int64_t nkeys, uint32_t kx, uin
 }  // namespace
 
 void plan_schedule(const gmm_preplan &pp, const uint64_t *sorted_mt, const float *xp, Chunk *chunks,
-                   uint8_t *qmask, uint32_t *counts_gpu, void *scratch,
+                   uint16_t *qmask, uint32_t *counts_gpu, void *scratch,
                    int64_t scratch_nbytes, hipStream_t stream)
 {
     const int64_t n = pp.plan_nmt;
