@@ -741,6 +741,100 @@ __device__ inline void t2m_sample(unsigned long long *__restrict__ acc, float *_
     }
 }
 
+// Fixed-point tod2map of one 4-sample group (t already scaled). When the (iy0, ix0) corners of
+// the group's evaluated samples span at most two rows and two columns, the group's bilinear
+// footprint lies in a 3x3 pixel window: its contributions are summed per pixel in registers and
+// added with one LDS atomic per touched pixel and plane (18 for a group that stays in one pixel
+// row, at most 27) instead of 12 per sample. A pixel's sum of at most four contributions, each
+// below 2^28 in magnitude, converts exactly to int32. Other groups deposit sample by sample.
+template<bool Periodic, bool HasResp, bool Small>
+__device__ inline void t2m_quad_fixed(unsigned long long *__restrict__ acc, const float2 *__restrict__ tab,
+                                      const float4 &y4, const float4 &x4, const float4 &a4, const float4 &t4,
+                                      const Geom &g, int ybase, int xbase, unsigned lxlim, const float (&tr)[4],
+                                      const float (&pr)[4], Checks &chk)
+{
+    const float yv[4] = { y4.x, y4.y, y4.z, y4.w }, xv[4] = { x4.x, x4.y, x4.z, x4.w };
+    const float av[4] = { a4.x, a4.y, a4.z, a4.w }, tv[4] = { t4.x, t4.y, t4.z, t4.w };
+    int ly[4], lx[4];
+    float dy[4], dx[4];
+    bool own[4];
+    int by = 64, bx = 1 << 20, ey = -1, ex = -1;          // row and column range of evaluated samples
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int iy0, ix0;
+        locate_op<Periodic>(yv[k], xv[k], g, chk, iy0, ix0, dy[k], dx[k]);
+        ly[k] = iy0 - ybase;
+        lx[k] = ix0 - xbase;
+        own[k] = (unsigned(ly[k]) < 64u) && (unsigned(lx[k]) < lxlim);
+        if (own[k]) {
+            by = min(by, ly[k]);
+            ey = max(ey, ly[k]);
+            bx = min(bx, lx[k]);
+            ex = max(ex, lx[k]);
+        }
+    }
+    if (ey < 0)
+        return;
+    if (ey - by <= 1 && ex - bx <= 1) {
+        float sum[3][3][3];
+#pragma unroll
+        for (int p = 0; p < 3; p++)
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    sum[p][i][j] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            float sn, cs;
+            sincos2<Small>(tab, av[k], sn, cs);
+            const float ts = HasResp ? tv[k] * tr[k] : tv[k];
+            const float ps = HasResp ? tv[k] * pr[k] : tv[k];
+            const float v[3] = { ts, ps * cs, ps * sn };
+            // Row weights at window rows 0..2 (the sample's rows are ly and ly + 1), likewise
+            // columns; a sample the block does not evaluate gets zero weights.
+            const bool ry = ly[k] != by, rx = lx[k] != bx;
+            const float wy0 = own[k] ? 1.0f - dy[k] : 0.0f, wy1 = own[k] ? dy[k] : 0.0f;
+            const float wx0 = 1.0f - dx[k], wx1 = dx[k];
+            const float r[3] = { ry ? 0.0f : wy0, ry ? wy0 : wy1, ry ? wy1 : 0.0f };
+            const float c[3] = { rx ? 0.0f : wx0, rx ? wx0 : wx1, rx ? wx1 : 0.0f };
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    const float rv = r[i] * v[p];
+#pragma unroll
+                    for (int j = 0; j < 3; j++)
+                        sum[p][i][j] = fmaf(rv, c[j], sum[p][i][j]);
+                }
+        }
+        // Window rows ey - by + 2 and columns ex - bx + 2 (2 or 3 each) lie inside the tile:
+        // ey + 1 <= 64 and ex + 1 <= lxlim <= HS - 1.
+        const int ni = ey - by + 2, nj = ex - bx + 2;
+        const int o = by * HS + bx;
+#pragma unroll
+        for (int p = 0; p < 3; p++)
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    if (i < ni && j < nj)
+                        dep_fixed(acc + p * HPLANE + o + i * HS + j, sum[p][i][j]);
+    }
+    else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (!own[k])
+                continue;
+            float sn, cs;
+            sincos2<Small>(tab, av[k], sn, cs);
+            const float ts = HasResp ? tv[k] * tr[k] : tv[k];
+            const float ps = HasResp ? tv[k] * pr[k] : tv[k];
+            deposit4<true>(acc, nullptr, ly[k] * HS + lx[k], dy[k], dx[k], ts, ps * cs, ps * sn);
+        }
+    }
+}
+
 // Largest |t| (times the gains) over the first 8 plan entries of [wb, we), for the calling
 // lane (lane 8*li + q reads quad q of entry wb + li).
 template<bool Vec, bool HasResp, typename TT>
@@ -841,18 +935,10 @@ __device__ inline void t2m_entries(unsigned long long *__restrict__ acc, float *
                 t4.y *= scale;
                 t4.z *= scale;
                 t4.w *= scale;
-                if (quad_small_angles(a4)) {
-                    t2m_sample<Periodic, HasResp, true, true>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, lxlim, tr[0], pr[0], chk);
-                    t2m_sample<Periodic, HasResp, true, true>(acc, facc, sct, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, lxlim, tr[1], pr[1], chk);
-                    t2m_sample<Periodic, HasResp, true, true>(acc, facc, sct, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, lxlim, tr[2], pr[2], chk);
-                    t2m_sample<Periodic, HasResp, true, true>(acc, facc, sct, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, lxlim, tr[3], pr[3], chk);
-                }
-                else {
-                    t2m_sample<Periodic, HasResp, true, false>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, lxlim, tr[0], pr[0], chk);
-                    t2m_sample<Periodic, HasResp, true, false>(acc, facc, sct, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, lxlim, tr[1], pr[1], chk);
-                    t2m_sample<Periodic, HasResp, true, false>(acc, facc, sct, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, lxlim, tr[2], pr[2], chk);
-                    t2m_sample<Periodic, HasResp, true, false>(acc, facc, sct, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, lxlim, tr[3], pr[3], chk);
-                }
+                if (quad_small_angles(a4))
+                    t2m_quad_fixed<Periodic, HasResp, true>(acc, sct, y4, x4, a4, t4, g, ybase, xbase, lxlim, tr, pr, chk);
+                else
+                    t2m_quad_fixed<Periodic, HasResp, false>(acc, sct, y4, x4, a4, t4, g, ybase, xbase, lxlim, tr, pr, chk);
             }
             else {
                 t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, lxlim, tr[0], pr[0], chk);
@@ -918,9 +1004,25 @@ __device__ inline void t2m_flush(float *__restrict__ lmap, const unsigned long l
         }
     }
     else if (h.own >= 0) {
-        // Groups of 4 pixels of a row, all planes; a group holding row 0, a column below catom
-        // or column W is done pixel by pixel in its thread, the others as float4
-        // read-modify-writes. All of a thread's float4 loads are issued first.
+        // Row 0 and columns [0, catom) of rows 1..63, which other blocks add to as well: one
+        // pixel per lane, atomically, so that the lanes of an atomic instruction share few cache
+        // lines (the 12 columns of a row are one 48-byte run).
+        const int nat = 64 + 63 * catom;                  // pixels per plane
+        for (int i = tid; i < 3 * nat; i += NT) {
+            const int pl = i / nat, k = i - pl * nat;
+            int r = 0, c = k;
+            if (k >= 64) {
+                r = 1 + (k - 64) / catom;
+                c = (k - 64) - (r - 1) * catom;
+            }
+            if (c == h.w)
+                continue;
+            unsafeAtomicAdd(lmap + h.own + pl * polstride + r * ystride + c,
+                            t2m_value(acc, facc, pl * HPLANE + r * HS + c, finite, inv_scale));
+        }
+        // Rows 1..63 from column catom on: groups of 4 pixels of a row, all planes, as float4
+        // read-modify-writes (all of a thread's loads issued first); a group that holds a column
+        // below catom or column W pixel by pixel.
         constexpr int NGRP = 3 * CELL_NPIX / 4;
         constexpr int NG = (NGRP + NT - 1) / NT;
         float4 old[NG];
@@ -939,6 +1041,8 @@ __device__ inline void t2m_flush(float *__restrict__ lmap, const unsigned long l
             if (g >= NGRP)
                 continue;
             const int pl = g >> 10, r = (g >> 4) & 63, c0 = (g & 15) << 2;
+            if (r == 0 || c0 + 3 < catom)
+                continue;
             const int kt = pl * HPLANE + r * HS + c0;
             float *gp = lmap + h.own + pl * polstride + r * ystride + c0;
             if (full[k]) {
@@ -953,13 +1057,9 @@ __device__ inline void t2m_flush(float *__restrict__ lmap, const unsigned long l
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
                     const int c = c0 + j;
-                    if (c == h.w)
+                    if (c < catom || c == h.w)
                         continue;
-                    const float v = t2m_value(acc, facc, kt + j, finite, inv_scale);
-                    if (r == 0 || c < catom)
-                        unsafeAtomicAdd(gp + j, v);
-                    else
-                        gp[j] += v;
+                    gp[j] += t2m_value(acc, facc, kt + j, finite, inv_scale);
                 }
             }
         }
