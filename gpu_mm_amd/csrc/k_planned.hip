@@ -808,18 +808,37 @@ __device__ inline void t2m_quad_fixed(unsigned long long *__restrict__ acc, cons
                         sum[p][i][j] = fmaf(rv, c[j], sum[p][i][j]);
                 }
         }
-        // Window rows ey - by + 2 and columns ex - bx + 2 (2 or 3 each) lie inside the tile:
-        // ey + 1 <= 64 and ex + 1 <= lxlim <= HS - 1.
-        const int ni = ey - by + 2, nj = ex - bx + 2;
+        // The window's rows 0..1 and columns 0..1 are always touched; row 2 when the evaluated
+        // samples span two rows (ey > by), column 2 likewise. Touched rows and columns lie inside
+        // the tile: ey + 1 <= 64 and ex + 1 <= lxlim <= HS - 1.
+        const bool row2 = ey > by, col2 = ex > bx;
         const int o = by * HS + bx;
 #pragma unroll
         for (int p = 0; p < 3; p++)
 #pragma unroll
-            for (int i = 0; i < 3; i++)
+            for (int i = 0; i < 2; i++)
 #pragma unroll
-                for (int j = 0; j < 3; j++)
-                    if (i < ni && j < nj)
-                        dep_fixed(acc + p * HPLANE + o + i * HS + j, sum[p][i][j]);
+                for (int j = 0; j < 2; j++)
+                    dep_fixed(acc + p * HPLANE + o + i * HS + j, sum[p][i][j]);
+        if (col2) {
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+                    dep_fixed(acc + p * HPLANE + o + i * HS + 2, sum[p][i][2]);
+        }
+        if (row2) {
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+                    dep_fixed(acc + p * HPLANE + o + 2 * HS + j, sum[p][2][j]);
+            if (col2) {
+#pragma unroll
+                for (int p = 0; p < 3; p++)
+                    dep_fixed(acc + p * HPLANE + o + 2 * HS + 2, sum[p][2][2]);
+            }
+        }
     }
     else {
 #pragma unroll

@@ -557,7 +557,6 @@ qmask_kernel(const uint64_t *__restrict__ mt, int64_t n, const float *__restrict
         s_nml = 0;
     __syncthreads();
     const int lane = lane_id();
-    const int nxcells = (nx + 63) >> 6;
     const int64_t total = 8 * n;
     // Line summary bits, OR-reduced over the entry's 32 samples.
     constexpr uint32_t HAS_L = 1, BAD_L = 2, HAS_Y = 4, BAD_Y = 8, HAS_R = 16, BAD_R = 32, TOUCH_L = 64,
@@ -574,9 +573,6 @@ qmask_kernel(const uint64_t *__restrict__ mt, int64_t n, const float *__restrict
             multi = (m & MFLAG) != 0;
             icell = uint32_t(m) & CELL_MASK;
             const int cy = int(icell >> 10), cx = int(icell & 1023u);
-            const uint32_t kl = (cx >= 1) ? cell_key(cy, cx - 1) : NO_CELL;
-            const uint32_t kll = (cx >= 2) ? cell_key(cy, cx - 2) : NO_CELL;
-            const uint32_t kr = (cx + 1 < nxcells) ? cell_key(cy, cx + 1) : NO_CELL;
             const int64_t s0 = int64_t(uint32_t(m >> ICL_SHIFT) & ICL_MASK) * LINE + 4 * q;
             // Samples past the end of a partial last line do not exist.
             const uint32_t nv = uint32_t(max<int64_t>(0, min<int64_t>(4, nsamp - s0)));
@@ -598,26 +594,20 @@ qmask_kernel(const uint64_t *__restrict__ mt, int64_t n, const float *__restrict
                     continue;
                 Pixel<float> p;
                 locate_pixel(yv[k], xv[k], ny, nx, Periodic, p);
-                uint32_t fk[4];
-                footprint_cells(p, fk);
-                const uint32_t o = fk[0];
+                // Cells of the footprint relative to Y: the owner (p.iy0, p.ix0) is in Y's cell
+                // row iff iy0 >> 6 == cy; its column offset is d, that of the ix1 corner d1.
+                const bool inrow = (p.iy0 >> 6) == cy;
+                const bool rowhit = inrow || ((p.iy1 >> 6) == cy);
+                const int d = (p.ix0 >> 6) - cx, d1 = (p.ix1 >> 6) - cx;
                 const bool fit = ((p.ix0 & 63) <= HALO_X - 2) && (p.ix0 + 1 < nx);
-                if (o == kl)
-                    sum |= fit ? HAS_L : (HAS_L | BAD_L);
-                if (o == icell) {
-                    sum |= fit ? HAS_Y : (HAS_Y | BAD_Y);
-                    own_y |= 1u << k;
-                }
-                if (o == kr) {
-                    sum |= fit ? HAS_R : (HAS_R | BAD_R);
-                    own_r |= 1u << k;
-                }
-#pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    if (fk[c] != NO_CELL && fk[c] == kl)
-                        sum |= TOUCH_L;
-                    if (fk[c] != NO_CELL && fk[c] == kll)
-                        sum |= TOUCH_LL;
+                // HAS/BAD bit pairs of the cells Y - 1, Y, Y + 1 sit at bit 2 * (d + 1).
+                if (inrow && d >= -1 && d <= 1)
+                    sum |= (fit ? 1u : 3u) << (2 * (d + 1));
+                own_y |= (inrow && d == 0) ? (1u << k) : 0u;
+                own_r |= (inrow && d == 1) ? (1u << k) : 0u;
+                if (rowhit) {
+                    sum |= (d == -1 || d1 == -1) ? TOUCH_L : 0u;
+                    sum |= (d == -2 || d1 == -2) ? TOUCH_LL : 0u;
                 }
             }
         }
