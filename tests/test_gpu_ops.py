@@ -727,3 +727,55 @@ def test_partial_last_line(gmm, orc, dev, nt, periodic):
         out = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
         b.tod2map(out, debug=True)
         assert l1rel(to_cpu(out.arr), mref32) < 1e-6 and l1rel(to_cpu(out.arr), mref64) < 1e-5
+
+
+# ------------------------------------------------------------------------------------------
+# Halo ownership rule near the right edge of the map: slow x scans (lines handed to the
+# left-hand cell's block) that cross into a last cell column narrower than the halo (nx % 64 =
+# 5 or 11 columns) and, in periodic mode, wrap around to column 0.
+
+@pytest.mark.parametrize("nx", [64 * 4 + 5, 64 * 3 + 11])
+@pytest.mark.parametrize("periodic", [False, True], ids=["nonperiodic", "periodic"])
+def test_halo_rule_right_edge(gmm, orc, dev, nx, periodic):
+    rng = np.random.default_rng(nx + periodic)
+    ndet, nt, ny = 16, 32 * 256, 130
+    t = np.arange(nt)
+    tri = 1.0 - 4.0 * np.abs((t / 600.0) % 1.0 - 0.5)           # period 600 samples, 0.27 px/sample
+    xp = np.empty((3, ndet, nt), F32)
+    for d in range(ndet):
+        xp[0, d] = 30 + 4.5 * d + 0.002 * t
+        if periodic:
+            xp[1, d] = (nx - 60 + 80 * (0.5 + 0.5 * tri) + 0.37 * d) % nx   # crosses x = nx and wraps
+        else:
+            xp[1, d] = nx - 80 + 78 * (0.5 + 0.5 * tri) - 0.11 * d          # up to x = nx - 2
+        xp[2, d] = rng.uniform(0, np.pi) + 0.001 * t
+    print(f"\n[halo right edge] nx={nx} periodic={periodic}", flush=True)
+    xg = to_gpu(xp, dev)
+    pre = gmm.PointingPrePlan(xg, ny, nx, periodic_xcoord=periodic)
+    plan = gmm.PointingPlan(pre, xg)
+    ref_mt, err = orc.plan_build(xp, ny, nx, periodic)
+    assert err == 0 and np.array_equal(plan.get_plan_mt(), ref_mt)
+    lp = gmm.LocalPixelization.make_rectangle(ny, nx, periodic_xcoord=periodic)
+    pix = orc.Pix.rectangle(ny, nx, periodic)
+    m = rng.standard_normal(3 * lp.npix).astype(F32)
+    mg = gmm.LocalMap(lp, to_gpu(m, dev))
+    ref32, _ = orc.map2tod(m, xp, pix)
+    ref64, _ = orc.map2tod(m.astype(np.float64), xp, pix, f64=True)
+    tg = torch.full((ndet, nt), 7.0, dtype=torch.float32, device=dev)
+    gmm.map2tod(tg, mg, xg, plan, debug=True)
+    assert l1rel(to_cpu(tg), ref32) < 1e-6 and l1rel(to_cpu(tg), ref64) < 1e-5
+    d = rng.standard_normal((ndet, nt)).astype(F32)
+    mref32, _ = orc.tod2map(d, xp, pix)
+    mref64, _ = orc.tod2map(d.astype(np.float64), xp, pix, f64=True)
+    out = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
+    gmm.tod2map(out, to_gpu(d, dev), xg, plan, debug=True)
+    got = to_cpu(out.arr)
+    assert l1rel(got, mref32) < 1e-6 and l1rel(got, mref64) < 1e-5
+    # Batched forms use the same ownership masks.
+    b = gmm.PlanBatch(lp, [plan], [xg], [tg])
+    b.map2tod(mg, debug=True)
+    assert l1rel(to_cpu(tg), ref32) < 1e-6
+    tg.copy_(to_gpu(d, dev))
+    out.arr.zero_()
+    b.tod2map(out, debug=True)
+    assert l1rel(to_cpu(out.arr), mref32) < 1e-6
