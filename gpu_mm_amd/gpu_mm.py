@@ -33,6 +33,8 @@ The 'plan' argument of map2tod/tod2map selects the implementation:
   'reference'   -> single-threaded CPU implementation (host arrays), for testing
 """
 
+import contextlib
+
 import numpy as np
 import torch
 
@@ -56,6 +58,22 @@ def set_errflag_check(flag):
 
 def get_errflag_check():
     return _errflag_check
+
+
+@contextlib.contextmanager
+def deferred_errflags():
+    """
+    Context manager: inside the block the GPU operators do not block on their error flags (as
+    after set_errflag_check(False)); errors accumulate on the plans and are raised by
+    PointingPlan.check_errflags(). The previous setting is restored on exit.
+    """
+    global _errflag_check
+    old = _errflag_check
+    _errflag_check = False
+    try:
+        yield
+    finally:
+        _errflag_check = old
 
 
 def _stream():

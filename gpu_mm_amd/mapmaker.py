@@ -132,13 +132,30 @@ class WeightedPointing:
         return _g.LocalMap(self.lpix, arr)
 
     def apply_normal(self, m, out=None):
-        """out = A^T N^-1 A m (out is zeroed first; a new array if None)."""
+        """
+        out = A^T N^-1 A m (out is zeroed first; a new array if None). The operator calls do not
+        block on their error flags (an iterative solver calls this many times): errors
+        accumulate on the plans and check_errflags() raises them.
+        """
         out = torch.zeros_like(m) if out is None else out.zero_()
         lm, lo = self._lmap(m), self._lmap(out)
-        for x, p, g, t in zip(self.xps, self.plans, self.g, self.work):
-            _g.map2tod(t, lm, x, p, response=g)
-            _g.tod2map(lo, t, x, p, response=g)
+        with _g.deferred_errflags():
+            for x, p, g, t in zip(self.xps, self.plans, self.g, self.work):
+                _g.map2tod(t, lm, x, p, response=g)
+                _g.tod2map(lo, t, x, p, response=g)
         return out
+
+    def check_errflags(self):
+        """Raises if an operator call of apply_normal() reported an error since the last check
+        (every plan's flags are read and cleared; the first error is raised)."""
+        first = None
+        for p in self.plans:
+            try:
+                p.check_errflags("WeightedPointing")
+            except RuntimeError as e:
+                first = e if first is None else first
+        if first is not None:
+            raise first
 
     def rhs(self, tods):
         """A^T N^-1 d for the list of TODs d."""
@@ -213,13 +230,33 @@ class BlockPreconditioner:
             dk = m[:, k, k]
             inv[:, k, k] = torch.where(diag & (dk > 0), 1.0 / dk.clamp(min=1e-300), inv[:, k, k])
         self.nfull, self.ndiag = int(full.sum()), int(diag.sum())
-        self.minv = inv.permute(1, 2, 0).contiguous().to(torch.float32)     # (3, 3, npix)
+        minv = inv.permute(1, 2, 0).to(torch.float32)    # (3, 3, npix), pixels in cell order
+        # When component p of every pixel sits at p * polstride + j with j covering
+        # [0, polstride) exactly (the dense rectangle), the inverse is stored in that order and
+        # applied to the (3, polstride) view of the map: no gather or scatter per application.
+        ps = int(wp.lpix.polstride)
+        self.planar = (3 * ps == wp.nmap and self.idx.shape[1] == ps
+                       and int(self.idx[0].min()) >= 0 and int(self.idx[0].max()) < ps)
+        if self.planar:
+            self.minv = torch.empty_like(minv)
+            self.minv[:, :, self.idx[0]] = minv
+        else:
+            self.minv = minv.contiguous()
 
     def __call__(self, r, out=None):
         # The pixel index covers every element of the local map (the cells tile it exactly).
         out = torch.empty_like(r) if out is None else out
-        v = r[self.idx]                                   # (3, npix)
-        out[self.idx] = (self.minv * v.unsqueeze(0)).sum(dim=1)
+        if self.planar:
+            v, o = r.view(3, -1), out.view(3, -1)
+        else:
+            v, o = r[self.idx], torch.empty((3, self.idx.shape[1]), dtype=r.dtype, device=r.device)
+        mi = self.minv
+        for i in range(3):
+            torch.mul(mi[i, 0], v[0], out=o[i])
+            o[i].addcmul_(mi[i, 1], v[1])
+            o[i].addcmul_(mi[i, 2], v[2])
+        if not self.planar:
+            out[self.idx] = o
         return out
 
 
@@ -227,17 +264,29 @@ class BlockPreconditioner:
 # Preconditioned conjugate gradient.
 
 def dot64(a, b):
-    """Inner product of two FP32 vectors, accumulated in FP64; a Python float."""
-    return float(torch.dot(a.double(), b.double()))
+    """Inner product of two FP32 vectors: FP32 products summed in FP64; a Python float."""
+    return float(torch.sum(a * b, dtype=torch.float64))
 
 
-def pcg(apply_a, b, precond=None, x0=None, niter=50, rtol=0.0, callback=None):
+def _dot64_dev(a, b):
+    """Inner product of two FP32 vectors (FP32 products summed in FP64), as a 0-d FP64 tensor."""
+    return torch.sum(a * b, dtype=torch.float64)
+
+
+def pcg(apply_a, b, precond=None, x0=None, niter=50, rtol=0.0, callback=None, check=None):
     """
     Solves A x = b (A symmetric positive definite, given as apply_a(x, out)) by preconditioned
     conjugate gradients. Vectors are FP32; inner products, step scalars and residual norms are
     FP64. Returns (x, history) with history[k] = ||r_k|| / ||b|| (k = 0..iterations run); stops
     after niter iterations or when the relative residual falls below rtol.
+
+    The scalars stay on the device: without rtol and callback, the iterations run without a
+    host synchronization and the history is read once at the end. A step whose curvature p.Ap
+    is not positive (breakdown, e.g. an exactly zero residual) leaves x unchanged: with rtol or
+    a callback the iteration stops there, otherwise the remaining iterations keep x as it is.
+    check, if given, is called once after the iterations (e.g. WeightedPointing.check_errflags).
     """
+    host_checks = (rtol > 0.0) or (callback is not None)
     x = torch.zeros_like(b) if x0 is None else x0.clone()
     r = b.clone()
     ap = torch.empty_like(b)
@@ -246,28 +295,36 @@ def pcg(apply_a, b, precond=None, x0=None, niter=50, rtol=0.0, callback=None):
         r -= ap
     z = precond(r) if precond is not None else r.clone()
     p = z.clone()
-    rz = dot64(r, z)
-    bnorm = np.sqrt(dot64(b, b))
-    hist = [np.sqrt(dot64(r, r)) / bnorm]
+    rz = _dot64_dev(r, z)
+    bnorm = float(torch.sqrt(_dot64_dev(b, b)))
+    rr = [_dot64_dev(r, r)]
     for k in range(niter):
         apply_a(p, ap)
-        pap = dot64(p, ap)
-        if not (pap > 0.0):
-            break
-        alpha = rz / pap
-        x.add_(p, alpha=alpha)
-        r.add_(ap, alpha=-alpha)
-        hist.append(np.sqrt(dot64(r, r)) / bnorm)
-        if callback is not None:
-            callback(k, x, hist[-1])
-        if hist[-1] < rtol:
-            break
+        pap = _dot64_dev(p, ap)
+        ok = pap > 0.0
+        alpha = torch.where(ok, rz / torch.where(ok, pap, 1.0), 0.0)
+        a32 = alpha.to(torch.float32)
+        x.addcmul_(p, a32)
+        r.addcmul_(ap, -a32)
+        rr.append(_dot64_dev(r, r))
+        if host_checks:
+            if not bool(ok):
+                rr.pop()
+                break
+            res = float(torch.sqrt(rr[-1])) / bnorm
+            if callback is not None:
+                callback(k, x, res)
+            if res < rtol:
+                break
         if precond is not None:
             precond(r, out=z)
         else:
             z.copy_(r)
-        rz_new = dot64(r, z)
-        beta = rz_new / rz
+        rz_new = _dot64_dev(r, z)
+        beta = torch.where(rz > 0.0, rz_new / torch.where(rz > 0.0, rz, 1.0), 0.0)
         rz = rz_new
-        p.mul_(beta).add_(z)
+        p.mul_(beta.to(torch.float32)).add_(z)
+    hist = (torch.sqrt(torch.stack(rr)) / bnorm).cpu().tolist()
+    if check is not None:
+        check()
     return x, hist

@@ -241,3 +241,19 @@ def test_pcg_recovers_smooth_map(gmm, dev):
     assert int(mask.sum()) > 10000
     assert errs[-1] < errs[0]
     assert errs[-1] < 5e-3
+
+
+def test_apply_normal_defers_errors(problem, gmm, dev):
+    """apply_normal() does not block on the error flags; check_errflags() raises what its calls
+    reported (here: a local map without the cells of the lower half of the sky)."""
+    pr = problem
+    cells = [(cy, cx) for cy in range(NY // 128) for cx in range(NX // 64)]
+    lp = gmm.LocalPixelization.make_cells(NY, NX, cells, periodic_xcoord=False)
+    wp = pr["mm"].WeightedPointing(lp, pr["plans"], pr["xps"], pr["ninv"])
+    m = torch.ones(3 * lp.npix, dtype=torch.float32, device=dev)
+    assert gmm.get_errflag_check()
+    wp.apply_normal(m)                       # returns without raising
+    assert gmm.get_errflag_check()
+    with pytest.raises(RuntimeError, match="not in LocalPixelization"):
+        wp.check_errflags()
+    wp.check_errflags()                      # the check cleared the flags

@@ -3,6 +3,7 @@ convergence to the exact solution of a random symmetric positive definite system
 without a Jacobi preconditioner, and FP64 scalars."""
 
 import numpy as np
+import pytest
 import torch
 
 from gpu_mm_amd import mapmaker
@@ -58,3 +59,32 @@ def test_pcg_zero_rhs_and_initial_guess():
     x, hist = mapmaker.pcg(apply_a, b, x0=x_true, niter=5)
     assert hist[0] < 1e-5
     assert torch.allclose(x, x_true, atol=1e-4)
+
+
+def test_pcg_breakdown_without_host_checks():
+    """A = 2 I: the first step solves the system exactly (r = 0); the following iterations have
+    zero curvature and, run without host checks, leave x unchanged (no NaN in x or history)."""
+    n = 16
+    b = torch.from_numpy(np.random.default_rng(5).standard_normal(n).astype(np.float32))
+
+    def apply_a(x, out):
+        torch.mul(x, 2.0, out=out)
+        return out
+
+    x, hist = mapmaker.pcg(apply_a, b, niter=6)
+    assert len(hist) == 7
+    assert all(np.isfinite(h) for h in hist)
+    assert hist[1] < 1e-7 and hist[-1] == hist[1]
+    assert torch.equal(x, b * 0.5)
+
+
+def test_deferred_errflags_restores_setting():
+    import gpu_mm_amd as gmm
+    assert gmm.get_errflag_check()
+    with gmm.deferred_errflags():
+        assert not gmm.get_errflag_check()
+    assert gmm.get_errflag_check()
+    with pytest.raises(ValueError):
+        with gmm.deferred_errflags():
+            raise ValueError("inside")
+    assert gmm.get_errflag_check()
