@@ -585,6 +585,7 @@ int gmm_plan_create(gmm_plan **out, const gmm_preplan *pp, const float *xpointin
             plan->nmlines = h[1];
             GMM_REQUIRE(plan->nchunks <= pp->max_nchunks,
                         "PointingPlan constructor: work list larger than its bound (internal error)");
+            order_chunks(*pp, plan->chunks, plan->nchunks, a, b, scratch, scratch_nbytes, stream);
         }
         catch (...) {
             delete plan;

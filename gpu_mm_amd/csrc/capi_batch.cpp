@@ -335,10 +335,12 @@ int gmm_batch_create(gmm_batch **out, const gmm_lpix *lpix, int32_t ntod, const 
             }
             b->tods.push_back(t);
 
-            // Cell runs of this plan: its chunks, merged per cell.
+            // Cell runs of this plan: its chunks (stored largest first) back in plan order, merged
+            // per cell.
             std::vector<Chunk> ch(size_t(pl->nchunks));
             if (!ch.empty())
                 GMM_HIP(hipMemcpy(ch.data(), pl->chunks, ch.size() * sizeof(Chunk), hipMemcpyDeviceToHost));
+            std::sort(ch.begin(), ch.end(), [](const Chunk &x, const Chunk &y) { return x.begin < y.begin; });
             for (const Chunk &c : ch) {
                 if (!b->runs.empty() && b->runs.back().tod == uint32_t(i) && b->runs.back().icell == c.icell
                     && b->runs.back().end == c.begin)

