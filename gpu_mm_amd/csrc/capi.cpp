@@ -201,11 +201,7 @@ int bits_for(int64_t n)
 }
 
 }  // namespace
-}  // namespace gmm
 
-using namespace gmm;
-
-namespace {
 // check_response(), src_lib/check_arguments.cu:171-180: response is float[2][ndet] (row 0: T
 // response, row 1: polarization response) and ndet must divide nsamp.
 void check_response(const float *response_gpu, int64_t ndet, int64_t nsamp, const char *where)
@@ -219,7 +215,10 @@ void check_response(const float *response_gpu, int64_t ndet, int64_t nsamp, cons
     if (nsamp / ndet > int64_t(UINT32_MAX))
         fail(std::string(where) + ": samples per detector must be < 2^32");
 }
-}  // namespace
+
+}  // namespace gmm
+
+using namespace gmm;
 
 namespace {
 // Arguments of the 16-bit TOD entry points.

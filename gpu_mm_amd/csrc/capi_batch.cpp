@@ -54,6 +54,7 @@ struct gmm_batch
     BatchTod *tods_dev = nullptr;
     uint32_t *words = nullptr;      // [0,2) map2tod queue, [2,4) tod2map queue, [4] error word
     DeviceTables m2t, t2m;
+    bool has_resp = false;          // every TOD has per-detector gains (gmm_batch_set_response)
     int64_t generation = 0;
 
     ~gmm_batch()
@@ -188,6 +189,7 @@ BatchLaunch launch_of(const gmm_batch &b, const DeviceTables &t, int op)
     bl.err_word = b.words + 4;
     bl.grid = t.grid;
     bl.threads = t.threads;
+    bl.has_resp = b.has_resp;
     return bl;
 }
 
@@ -324,6 +326,7 @@ int gmm_batch_create(gmm_batch **out, const gmm_lpix *lpix, int32_t ntod, const 
             t.plan_mt = pl->plan_mt;
             t.qmask = pl->qmask;
             t.nsamp = nsamp[i];
+            t.resp_ndet = t.resp_nper = 1;
             if (q16) {
                 require_ptr(tod_scale_gpu[i], ws.c_str(), "tod_scale");
                 const int64_t nr = tod_nrows[i];
@@ -370,6 +373,37 @@ int gmm_batch_create(gmm_batch **out, const gmm_lpix *lpix, int32_t ntod, const 
 }
 
 void gmm_batch_destroy(gmm_batch *b) { delete b; }
+
+int gmm_batch_set_response(gmm_batch *b, const float *const *response_gpu, const int64_t *resp_ndet)
+{
+    return guarded([&] {
+        const char *where = "batch set_response";
+        require_ptr(b, where, "batch");
+        std::vector<BatchTod> tods = b->tods;
+        if (response_gpu) {
+            require_ptr(resp_ndet, where, "resp_ndet");
+            if (b->tod_type != TOD_F32)
+                fail(std::string(where) + ": per-detector gains are supported for float32 TODs only");
+            for (size_t i = 0; i < tods.size(); i++) {
+                const std::string ws = std::string(where) + " (TOD " + std::to_string(i) + ")";
+                check_response(response_gpu[i], resp_ndet[i], tods[i].nsamp, ws.c_str());
+                tods[i].resp = response_gpu[i];
+                tods[i].resp_ndet = uint32_t(resp_ndet[i]);
+                tods[i].resp_nper = uint32_t(tods[i].nsamp / resp_ndet[i]);
+            }
+        }
+        else {
+            for (BatchTod &t : tods) {
+                t.resp = nullptr;
+                t.resp_ndet = t.resp_nper = 1;
+            }
+        }
+        GMM_HIP(hipMemcpy(b->tods_dev, tods.data(), tods.size() * sizeof(BatchTod), hipMemcpyHostToDevice));
+        b->tods = tods;
+        b->has_resp = (response_gpu != nullptr);
+        b->generation++;
+    });
+}
 
 int gmm_batch_configure(gmm_batch *b, const gmm_batch_config *config)
 {

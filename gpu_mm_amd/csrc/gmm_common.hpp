@@ -251,8 +251,10 @@ struct BatchTod
     const float *scale;           // 16-bit TODs: float[nsamp / scale_nper], else null
     const uint64_t *plan_mt;      // the TOD's sorted plan entries
     const uint16_t *qmask;        // the plan's owned-quad masks
+    const float *resp;            // per-detector gains float[2][resp_ndet], or null
     int64_t nsamp;
     uint32_t scale_nper;          // samples per scale row
+    uint32_t resp_ndet, resp_nper; // detectors and samples per detector of resp
     uint32_t pad;
 };
 

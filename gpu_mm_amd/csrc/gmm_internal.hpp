@@ -105,6 +105,8 @@ void throw_errflags(uint32_t err, const char *where, uint32_t ignore = 0);
 // Argument checks (src_lib/check_arguments.cu:137-156).
 void check_nsamp(int64_t nsamp, const char *where);
 void check_nsamp_quads(int64_t nsamp, const char *where);
+// Per-detector gains float[2][ndet] of a TOD of nsamp samples (ndet > 0 divides nsamp).
+void check_response(const float *response_gpu, int64_t ndet, int64_t nsamp, const char *where);
 void check_npix_global(int64_t n, const char *what, const char *where);
 
 // ------------------------------------------------------------------------------------------
@@ -211,6 +213,7 @@ struct BatchLaunch
     uint32_t *err_word = nullptr;
     int64_t grid = 0;
     int threads = 512;
+    bool has_resp = false;      // every TOD has per-detector gains (FP32 TODs only)
 };
 
 void launch_batched_map2tod(const BatchLaunch &bl, const float *lmap, const gmm_lpix &lp, int tod_type, bool partial,

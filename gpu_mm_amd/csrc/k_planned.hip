@@ -1298,10 +1298,10 @@ __device__ inline void batch_wave_segments(const BatchSeg *__restrict__ segs, co
 
 __device__ inline Gains batch_gains(const BatchTod &td)
 {
-    return Gains{ nullptr, td.scale, 1u, 1u, td.scale_nper };
+    return Gains{ td.resp, td.scale, td.resp_ndet, td.resp_nper, td.scale_nper };
 }
 
-template<bool Periodic, typename TT, int NT>
+template<bool Periodic, typename TT, int NT, bool HasResp = false>
 __global__ void __launch_bounds__(NT, 4)
 batched_map2tod_kernel(const float *__restrict__ lmap, const BatchTod *__restrict__ tods,
                        const BatchSeg *__restrict__ segs, const BatchItem *__restrict__ items,
@@ -1333,7 +1333,7 @@ batched_map2tod_kernel(const float *__restrict__ lmap, const BatchTod *__restric
         const uint32_t we = uint32_t((uint64_t(it.nent) * (wave + 1)) / (NT / WAVE));
         batch_wave_segments(segs, it, wb, we, [&](const BatchSeg &sg, uint32_t b, uint32_t e) {
             const BatchTod td = tods[sg.tod];
-            m2t_entries<Periodic, true, false, TT>(static_cast<TT *>(td.tod), td.xp, td.plan_mt, td.qmask, td.nsamp,
+            m2t_entries<Periodic, true, HasResp, TT>(static_cast<TT *>(td.tod), td.xp, td.plan_mt, td.qmask, td.nsamp,
                                                    b, e, tile, sct, g, ybase, xbase, ba.accum, batch_gains(td), chk,
                                                    err);
         });
@@ -1345,7 +1345,7 @@ batched_map2tod_kernel(const float *__restrict__ lmap, const BatchTod *__restric
     report_err(err | chk.bits(g), ba.err_word);
 }
 
-template<bool Periodic, typename TT>
+template<bool Periodic, typename TT, bool HasResp = false>
 __global__ void __launch_bounds__(T2M_THREADS)
 batched_tod2map_kernel(float *__restrict__ lmap, const BatchTod *__restrict__ tods,
                        const BatchSeg *__restrict__ segs, const BatchItem *__restrict__ items,
@@ -1387,7 +1387,7 @@ batched_tod2map_kernel(float *__restrict__ lmap, const BatchTod *__restrict__ to
             batch_wave_segments(segs, it, wb, we, [&](const BatchSeg &sg, uint32_t b, uint32_t e) {
                 if (first) {
                     const BatchTod td = tods[sg.tod];
-                    m = t2m_probe<true, false, TT>(static_cast<const TT *>(td.tod), td.plan_mt, td.nsamp, b, e,
+                    m = t2m_probe<true, HasResp, TT>(static_cast<const TT *>(td.tod), td.plan_mt, td.nsamp, b, e,
                                                    batch_gains(td));
                     first = false;
                 }
@@ -1405,7 +1405,7 @@ batched_tod2map_kernel(float *__restrict__ lmap, const BatchTod *__restrict__ to
             [&](bool fin, float scale, float &mrun, float &tsum, Checks &chk) {
                 batch_wave_segments(segs, it, wb, we, [&](const BatchSeg &sg, uint32_t b, uint32_t e) {
                     const BatchTod td = tods[sg.tod];
-                    t2m_entries<Periodic, true, false, TT>(acc, facc, static_cast<const TT *>(td.tod), td.xp,
+                    t2m_entries<Periodic, true, HasResp, TT>(acc, facc, static_cast<const TT *>(td.tod), td.xp,
                                                            td.plan_mt, td.qmask, td.nsamp, b, e, sct, g, ybase, xbase,
                                                            batch_gains(td), fin, scale, mrun, tsum, chk, err);
                 });
@@ -1543,6 +1543,30 @@ void batched_t2m_typed(const BatchLaunch &bl, float *lmap, const int64_t *cell_o
     GMM_HIP(hipGetLastError());
 }
 
+// Batched operators with per-detector gains (float32 TODs).
+void batched_m2t_resp(const BatchLaunch &bl, const float *lmap, const int64_t *cell_offsets, const BatchArgs &ba,
+                      bool periodic, hipStream_t stream)
+{
+    const unsigned grid = batch_grid(bl);
+    if (bl.threads == 256) {
+        auto k = periodic ? batched_map2tod_kernel<true, float, 256, true> : batched_map2tod_kernel<false, float, 256, true>;
+        k<<<grid, 256, 0, stream>>>(lmap, bl.tods, bl.segs, bl.items, cell_offsets, ba);
+    }
+    else {
+        auto k = periodic ? batched_map2tod_kernel<true, float, 512, true> : batched_map2tod_kernel<false, float, 512, true>;
+        k<<<grid, 512, 0, stream>>>(lmap, bl.tods, bl.segs, bl.items, cell_offsets, ba);
+    }
+    GMM_HIP(hipGetLastError());
+}
+
+void batched_t2m_resp(const BatchLaunch &bl, float *lmap, const int64_t *cell_offsets, const BatchArgs &ba,
+                      bool periodic, hipStream_t stream)
+{
+    auto k = periodic ? batched_tod2map_kernel<true, float, true> : batched_tod2map_kernel<false, float, true>;
+    k<<<batch_grid(bl), T2M_THREADS, 0, stream>>>(lmap, bl.tods, bl.segs, bl.items, cell_offsets, ba);
+    GMM_HIP(hipGetLastError());
+}
+
 }  // namespace
 
 // ==========================================================================================
@@ -1575,9 +1599,15 @@ void launch_batched_map2tod(const BatchLaunch &bl, const float *lmap, const gmm_
     if (bl.nitems <= 0)
         return;
     GMM_REQUIRE(bl.threads == 256 || bl.threads == 512, "batched map2tod: threads per block must be 256 or 512");
+    GMM_REQUIRE(!bl.has_resp || tod_type == TOD_F32, "batched map2tod: per-detector gains need float32 TODs");
     const BatchArgs ba = batch_args(bl, lp, partial, accum);
     switch (tod_type) {
-    case TOD_F32: batched_m2t_typed<float>(bl, lmap, lp.offsets_gpu, ba, lp.periodic_xcoord, stream); break;
+    case TOD_F32:
+        if (bl.has_resp)
+            batched_m2t_resp(bl, lmap, lp.offsets_gpu, ba, lp.periodic_xcoord, stream);
+        else
+            batched_m2t_typed<float>(bl, lmap, lp.offsets_gpu, ba, lp.periodic_xcoord, stream);
+        break;
     case TOD_F16: batched_m2t_typed<_Float16>(bl, lmap, lp.offsets_gpu, ba, lp.periodic_xcoord, stream); break;
     case TOD_BF16: batched_m2t_typed<__bf16>(bl, lmap, lp.offsets_gpu, ba, lp.periodic_xcoord, stream); break;
     default: fail("batched map2tod: unknown TOD element type");
@@ -1590,8 +1620,14 @@ void launch_batched_tod2map(const BatchLaunch &bl, float *lmap, const gmm_lpix &
     if (bl.nitems <= 0)
         return;
     const BatchArgs ba = batch_args(bl, lp, partial, false);
+    GMM_REQUIRE(!bl.has_resp || tod_type == TOD_F32, "batched tod2map: per-detector gains need float32 TODs");
     switch (tod_type) {
-    case TOD_F32: batched_t2m_typed<float>(bl, lmap, lp.offsets_gpu, ba, lp.periodic_xcoord, stream); break;
+    case TOD_F32:
+        if (bl.has_resp)
+            batched_t2m_resp(bl, lmap, lp.offsets_gpu, ba, lp.periodic_xcoord, stream);
+        else
+            batched_t2m_typed<float>(bl, lmap, lp.offsets_gpu, ba, lp.periodic_xcoord, stream);
+        break;
     case TOD_F16: batched_t2m_typed<_Float16>(bl, lmap, lp.offsets_gpu, ba, lp.periodic_xcoord, stream); break;
     case TOD_BF16: batched_t2m_typed<__bf16>(bl, lmap, lp.offsets_gpu, ba, lp.periodic_xcoord, stream); break;
     default: fail("batched tod2map: unknown TOD element type");

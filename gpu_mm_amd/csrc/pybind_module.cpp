@@ -467,6 +467,33 @@ struct Batch
     }
     ~Batch() { gmm_batch_destroy(h); }
 
+    // Per-detector gains of every TOD (a list of (2, ndet) float32 arrays), or None.
+    void set_response(py::object responses)
+    {
+        const char *where = "PlanBatch.set_response";
+        py::list keep;
+        std::vector<const float *> rp;
+        std::vector<int64_t> nd;
+        if (!responses.is_none()) {
+            py::list l(responses);
+            if (int64_t(py::len(l)) != info().ntod)
+                raise(std::string(where) + ": expected one response array per TOD");
+            for (size_t i = 0; i < py::len(l); i++) {
+                Arr r = response_arr(l[i], where);
+                rp.push_back(static_cast<const float *>(r.ptr));
+                nd.push_back(r.shape[1]);
+                keep.append(l[i]);
+            }
+        }
+        {
+            py::gil_scoped_release nogil;
+            check(gmm_batch_set_response(h, responses.is_none() ? nullptr : rp.data(),
+                                         responses.is_none() ? nullptr : nd.data()));
+        }
+        resp_refs = keep;
+    }
+    py::list resp_refs;       // the response arrays the batch points into
+
     gmm_batch_info info() const
     {
         gmm_batch_info i;
@@ -794,6 +821,7 @@ PYBIND11_MODULE(_gpu_mm_amd_py, m)
             py::gil_scoped_release nogil;
             check(gmm_batch_configure(b.h, &c));
         })
+        .def("set_response", &Batch::set_response, py::arg("responses"))
         .def("map2tod", [](Batch &b, py::object local_map, bool partial, bool accum, bool check_errors,
                            uintptr_t stream) {
             Arr m = b.map(local_map, "PlanBatch.map2tod");

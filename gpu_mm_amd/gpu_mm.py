@@ -643,6 +643,15 @@ class PlanBatch:
     def configure(self, config):
         self._b.configure(config)
 
+    def set_response(self, responses):
+        """
+        Per-detector gains for every bound TOD (float32 TODs only): responses[i] is a float32
+        (2, ndet_i) GPU array as in map2tod(..., response=...), with ndet_i dividing TOD i's
+        sample count; the batched operators then apply them as the per-TOD calls do. None removes
+        them. Graphs captured before the call must be captured again.
+        """
+        self._b.set_response(None if responses is None else list(responses))
+
     def _arr(self, local_map):
         if isinstance(local_map, LocalMap):
             if not (local_map.pixelization == self.pixelization):

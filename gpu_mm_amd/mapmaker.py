@@ -123,6 +123,14 @@ class WeightedPointing:
         self.g = [torch.stack([torch.sqrt(w), torch.sqrt(w)]).contiguous() for w in self.ninv]
         self.w2 = [torch.stack([w, w]).contiguous() for w in self.ninv]
         self.work = [torch.empty(tuple(x.shape[1:]), dtype=torch.float32, device=x.device) for x in self.xps]
+        # The normal operator runs as one batched map2tod and one batched tod2map over all TODs
+        # (the gains as per-detector responses); TODs that cannot be batched (a sample count
+        # not divisible by 4) are run one by one.
+        try:
+            self.batch = _g.PlanBatch(lpix, self.plans, self.xps, self.work)
+            self.batch.set_response(self.g)
+        except RuntimeError:
+            self.batch = None
 
     @property
     def nmap(self):
@@ -140,18 +148,25 @@ class WeightedPointing:
         out = torch.zeros_like(m) if out is None else out.zero_()
         lm, lo = self._lmap(m), self._lmap(out)
         with _g.deferred_errflags():
-            for x, p, g, t in zip(self.xps, self.plans, self.g, self.work):
-                _g.map2tod(t, lm, x, p, response=g)
-                _g.tod2map(lo, t, x, p, response=g)
+            if self.batch is not None:
+                self.batch.map2tod(lm)
+                self.batch.tod2map(lo)
+            else:
+                for x, p, g, t in zip(self.xps, self.plans, self.g, self.work):
+                    _g.map2tod(t, lm, x, p, response=g)
+                    _g.tod2map(lo, t, x, p, response=g)
         return out
 
     def check_errflags(self):
         """Raises if an operator call of apply_normal() reported an error since the last check
         (every plan's flags are read and cleared; the first error is raised)."""
         first = None
-        for p in self.plans:
+        checks = [p.check_errflags for p in self.plans]
+        if self.batch is not None:
+            checks.append(self.batch.check_errflags)
+        for c in checks:
             try:
-                p.check_errflags("WeightedPointing")
+                c("WeightedPointing")
             except RuntimeError as e:
                 first = e if first is None else first
         if first is not None:

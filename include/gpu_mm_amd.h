@@ -307,6 +307,14 @@ int gmm_batch_create(gmm_batch **out, const gmm_lpix *lpix, int32_t ntod, const 
                      const float *const *xpointing_gpu, void *const *tod_gpu, const int64_t *nsamp,
                      int32_t tod_type, const float *const *tod_scale_gpu, const int64_t *tod_nrows,
                      const gmm_batch_config *config);
+
+/* No counterpart in the reference. Per-detector gains for every TOD of the batch, as in
+ * gmm_response_map2tod: response_gpu[i] is float[2][resp_ndet[i]] (row 0 multiplies T, row 1 Q
+ * and U) and resp_ndet[i] divides TOD i's nsamp; the batched operators then compute
+ * response_map2tod / response_tod2map per TOD. response_gpu == NULL removes the gains. Float32
+ * TODs only. The arrays must outlive the batch's use of them; graphs captured before the call
+ * must be captured again. */
+int gmm_batch_set_response(gmm_batch *b, const float *const *response_gpu, const int64_t *resp_ndet);
 void gmm_batch_destroy(gmm_batch *b);
 /* Rebuilds the work items for a new configuration (invalidates graphs captured before). */
 int gmm_batch_configure(gmm_batch *b, const gmm_batch_config *config);

@@ -292,3 +292,67 @@ def test_batch_argument_errors(batch_case, gmm, dev):
         b.configure(make_config(gmm, m2t_item_entries=16))
     with pytest.raises(RuntimeError, match="m2t_threads"):
         b.configure(make_config(gmm, m2t_threads=1024))
+
+
+@pytest.mark.parametrize("cfg", CONFIGS, ids=["default", "small-items", "persistent-split", "whole-cells"])
+def test_batch_response(batch_case, gmm, orc, dev, cfg):
+    """Per-detector gains (set_response): batched map2tod bit-identical to the plain response
+    map2tod of each TOD; batched tod2map within the response bar of the FP64 response oracle
+    summed over the TODs; set_response(None) restores the plain operators."""
+    c = batch_case
+    lp = c["lp"]
+    rng = np.random.default_rng(23)
+    resps = [rng.uniform(0.5, 1.5, (2, x.shape[1])).astype(F32) for x in c["xps"]]
+    rg = [torch.from_numpy(r).to(dev) for r in resps]
+    mg = gmm.LocalMap(lp, torch.from_numpy(c["m"]).to(dev))
+    outs = [torch.full(tuple(x.shape[1:]), 7.0, dtype=torch.float32, device=dev) for x in c["xps"]]
+    b = gmm.PlanBatch(lp, c["plans"], c["xps"], outs, config=make_config(gmm, **cfg))
+    b.set_response(rg)
+    b.map2tod(mg, debug=True)
+    for x, p, r, o in zip(c["xps"], c["plans"], rg, outs):
+        ref = torch.empty_like(o)
+        gmm.map2tod(ref, mg, x, p, response=r, debug=True)
+        assert torch.equal(o, ref)
+    for o, t in zip(outs, c["tods"]):
+        o.copy_(torch.from_numpy(t).to(dev))
+    out = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
+    b.tod2map(out, debug=True)
+    pix = oracle_pix(orc, c)
+    mref = np.zeros(3 * lp.npix)
+    for x, t, r in zip(c["xps"], c["tods"], resps):
+        mr, err = orc.response_tod2map(t, x.cpu().numpy(), r, pix)
+        assert err == 0
+        mref += mr
+    assert l1rel(to_cpu(out.arr), mref) < 1e-5       # the response oracle is FP64
+    # Without gains again: the plain batched map2tod.
+    b.set_response(None)
+    b.map2tod(mg, debug=True)
+    for x, p, o in zip(c["xps"], c["plans"], outs):
+        ref = torch.empty_like(o)
+        gmm.map2tod(ref, mg, x, p, debug=True)
+        assert torch.equal(o, ref)
+
+
+def test_batch_response_errors(batch_case, gmm, dev):
+    c = batch_case
+    lp = c["lp"]
+    outs = [torch.zeros(tuple(x.shape[1:]), dtype=torch.float32, device=dev) for x in c["xps"]]
+    b = gmm.PlanBatch(lp, c["plans"], c["xps"], outs)
+    good = [torch.ones((2, x.shape[1]), dtype=torch.float32, device=dev) for x in c["xps"]]
+    with pytest.raises(RuntimeError, match="one response array per TOD"):
+        b.set_response(good[:-1])
+    bad = list(good)
+    bad[1] = torch.ones((2, c["xps"][1].shape[1] * c["xps"][1].shape[2] + 1), dtype=torch.float32, device=dev)
+    with pytest.raises(RuntimeError, match="does not divide nsamp"):
+        b.set_response(bad)
+    with pytest.raises(RuntimeError, match=r"shape \(2, ndet\)"):
+        b.set_response([torch.ones((3, x.shape[1]), dtype=torch.float32, device=dev) for x in c["xps"]])
+    t16 = []
+    sc = []
+    for x in c["xps"]:
+        t, s = gmm.encode_tod16(torch.zeros(tuple(x.shape[1:]), dtype=torch.float32, device=dev))
+        t16.append(t)
+        sc.append(s)
+    b16 = gmm.PlanBatch(lp, c["plans"], c["xps"], t16, scales=sc)
+    with pytest.raises(RuntimeError, match="float32 TODs only"):
+        b16.set_response(good)
