@@ -423,7 +423,7 @@ def cg_leg(torch, gmm, synth, dev, niter=50):
         plans.append(p)
         ninv.append(torch.from_numpy((1.0 / var).astype(np.float32)).to(dev))
         tods.append(t)
-    wp = mm.WeightedPointing(lp, plans, xps, ninv)
+    wp = mm.WeightedPointing(lp, plans, xps, ninv, autotune=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     pc = mm.BlockPreconditioner(wp)
@@ -458,7 +458,8 @@ def cg_leg(torch, gmm, synth, dev, niter=50):
                normal_op_ms=round(ms_normal, 4), solve_ms=round(1e3 * t_solve, 2),
                ms_per_iteration=round(1e3 * t_solve / max(1, len(hist) - 1), 4),
                residual_history=[float(f"{h:.3e}") for h in hist[::5]],
-               preconditioner={"full_3x3_pixels": pc.nfull, "diagonal_pixels": pc.ndiag})
+               preconditioner={"full_3x3_pixels": pc.nfull, "diagonal_pixels": pc.ndiag},
+               normal_op_batch_config=(str(wp.batch.config) if wp.batch is not None else None))
     del wp, pc, b, x, xps, plans, tods
     torch.cuda.empty_cache()
     return out

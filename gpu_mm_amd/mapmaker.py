@@ -106,11 +106,13 @@ class WeightedPointing:
 
     ninv[i] is a float32 GPU tensor of per-detector inverse variances of TOD i (TOD i has shape
     (ndet_i, nt_i)). The weights enter the kernels as per-detector gains (response rows (g, g)):
-    A_g m = g A m with g = sqrt(ninv), so A^T N^-1 A = A_g^T A_g, evaluated with one map2tod and
-    one tod2map per TOD and no pass over the TODs in between.
+    A_g m = g A m with g = sqrt(ninv), so A^T N^-1 A = A_g^T A_g, evaluated with one batched
+    map2tod and one batched tod2map over all TODs and no pass over the TODs in between.
+    autotune=True times the batch configurations once at construction (PlanBatch.autotune) and
+    keeps the fastest.
     """
 
-    def __init__(self, lpix, plans, xpointings, ninv):
+    def __init__(self, lpix, plans, xpointings, ninv, autotune=False):
         if not (len(plans) == len(xpointings) == len(ninv)):
             raise RuntimeError("WeightedPointing: plans, xpointings and ninv must have equal length")
         self.lpix = lpix
@@ -131,6 +133,11 @@ class WeightedPointing:
             self.batch.set_response(self.g)
         except RuntimeError:
             self.batch = None
+        if autotune and self.batch is not None:
+            scratch = torch.zeros(self.nmap, dtype=torch.float32, device=self.xps[0].device)
+            with _g.deferred_errflags():
+                self.batch.autotune(_g.LocalMap(lpix, scratch))
+            self.batch.check_errflags("WeightedPointing")
 
     @property
     def nmap(self):
