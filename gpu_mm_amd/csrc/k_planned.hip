@@ -140,12 +140,21 @@ __device__ inline float4 load4n(const float *__restrict__ p, uint32_t nv)
 }
 
 
+// Four consecutive floats (16-byte aligned) with a non-temporal load.
+__device__ inline float4 load4_nt(const float *__restrict__ p)
+{
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    const f4v v = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // TOD element access, four consecutive samples at a time. TT is float, _Float16 or __bf16.
 // 16-bit TODs are always accessed with 8-byte vector operations (the host requires an 8-byte
 // aligned TOD), and multi-cell lines use the packed 16-bit global atomics. Whole-group stores
-// are non-temporal: map2tod's output is written once and not read back by the kernel, and
-// keeping it out of the caches leaves them to the pointing lines that neighbouring cells share
-// (measured: 5% faster map2tod; non-temporal pointing loads, which give up that sharing, are
+// are non-temporal: map2tod's output is written once and not read back by the kernel (measured:
+// 5% faster map2tod). map2tod's pointing loads are non-temporal too (load4_nt): with the x halo
+// a line crossing a vertical cell boundary is read by one block only, so the caches gain little
+// from keeping pointing lines (measured: 1.8% faster map2tod; before the halo such loads were
 // slower).
 template<typename TT, bool Vec>
 struct TodIO
@@ -600,9 +609,9 @@ __device__ inline void m2t_entries(TT *__restrict__ tod, const float *__restrict
                 continue;
             const int64_t s0 = int64_t(icl) * LINE + 4 * q;
             const uint32_t nv = uint32_t(min<int64_t>(4, nsamp - s0));   // < 4 only at the end of a TOD
-            const float4 y4 = load4n<Vec>(xp + s0, nv);
-            const float4 x4 = load4n<Vec>(xp + nsamp + s0, nv);
-            const float4 a4 = load4n<Vec>(xp + 2 * nsamp + s0, nv);
+            const float4 y4 = Vec ? load4_nt(xp + s0) : load4n<Vec>(xp + s0, nv);
+            const float4 x4 = Vec ? load4_nt(xp + nsamp + s0) : load4n<Vec>(xp + nsamp + s0, nv);
+            const float4 a4 = Vec ? load4_nt(xp + 2 * nsamp + s0) : load4n<Vec>(xp + 2 * nsamp + s0, nv);
             float tr[4], pr[4], sc[4];
             quad_resp<HasResp>(gn, uint32_t(s0), tr, pr, nv);
             quad_scale<TT>(gn, uint32_t(s0), sc);
