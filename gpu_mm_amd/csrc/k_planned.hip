@@ -793,10 +793,30 @@ __device__ inline void t2m_quad_fixed(unsigned long long *__restrict__ acc, cons
 #pragma unroll
                 for (int j = 0; j < 3; j++)
                     sum[p][i][j] = 0.0f;
+        // sin and cos of 2 alpha: from the table for sample 0 and, when every lane's angles
+        // stay within |2 alpha_k - 2 alpha_0| <= 0.0185 (the table's own reduction bound, where
+        // cos d = 1 - d^2/2 and sin d = d (1 - d^2/6) are exact to float rounding), for the other
+        // samples by angle addition from sample 0 (consecutive samples of one detector).
+        float s0, c0;
+        sincos2<Small>(tab, av[0], s0, c0);
+        const float x0 = 2.0f * av[0];
+        const float dmax = fmaxf(fmaxf(fabsf(2.0f * av[1] - x0), fabsf(2.0f * av[2] - x0)), fabsf(2.0f * av[3] - x0));
+        const bool near = Small && !__any(!(dmax <= 0.0185f));
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            float sn, cs;
-            sincos2<Small>(tab, av[k], sn, cs);
+            float sn = s0, cs = c0;
+            if (k > 0) {
+                if (near) {
+                    const float d = 2.0f * av[k] - x0;
+                    const float z = d * d;
+                    const float cd = fmaf(z, -0.5f, 1.0f);
+                    const float sd = d * fmaf(z, -0.16666667f, 1.0f);
+                    sn = fmaf(s0, cd, c0 * sd);
+                    cs = fmaf(c0, cd, -(s0 * sd));
+                }
+                else
+                    sincos2<Small>(tab, av[k], sn, cs);
+            }
             const float ts = HasResp ? tv[k] * tr[k] : tv[k];
             const float ps = HasResp ? tv[k] * pr[k] : tv[k];
             const float v[3] = { ts, ps * cs, ps * sn };
