@@ -955,10 +955,16 @@ __device__ inline void t2m_entries(unsigned long long *__restrict__ acc, float *
             const int64_t s0 = int64_t(icl) * LINE + 4 * q;
             // Past the end of the TOD (nv < 4) the samples read as zeros: they deposit nothing.
             const uint32_t nv = uint32_t(min<int64_t>(4, nsamp - s0));
-            const float4 y4 = load4n<Vec>(xp + s0, nv);
-            const float4 x4 = load4n<Vec>(xp + nsamp + s0, nv);
-            const float4 a4 = load4n<Vec>(xp + 2 * nsamp + s0, nv);
-            float4 t4 = IO::load_n(tod + s0, nv);
+            // Non-temporal: the pointing and TOD stream through once, while the cell's map lines
+            // (touched at block start by t2m_warm_cell) should stay cached for the flush.
+            const float4 y4 = Vec ? load4_nt(xp + s0) : load4n<Vec>(xp + s0, nv);
+            const float4 x4 = Vec ? load4_nt(xp + nsamp + s0) : load4n<Vec>(xp + nsamp + s0, nv);
+            const float4 a4 = Vec ? load4_nt(xp + 2 * nsamp + s0) : load4n<Vec>(xp + 2 * nsamp + s0, nv);
+            float4 t4;
+            if constexpr (std::is_same<TT, float>::value)
+                t4 = Vec ? load4_nt(tod + s0) : IO::load_n(tod + s0, nv);
+            else
+                t4 = IO::load_n(tod + s0, nv);
             float tr[4], pr[4], sc[4];
             quad_resp<HasResp>(gn, uint32_t(s0), tr, pr, nv);
             quad_scale<TT>(gn, uint32_t(s0), sc);
@@ -1020,6 +1026,24 @@ __device__ inline float t2m_value(const unsigned long long *__restrict__ acc, co
                                   bool finite, float inv_scale)
 {
     return finite ? float((long long)acc[k]) * inv_scale : facc[k];
+}
+
+// Touches the cell's map lines (one float per 32-byte sector) at block start, so that the
+// flush's read-modify-writes find them in the caches: their loads overlap the probe, and the
+// accumulation's pointing and TOD loads are non-temporal (measured: tod2map 2-3% faster). The
+// loaded values only feed an empty asm statement that keeps the loads.
+template<int NT>
+__device__ inline void t2m_warm_cell(const float *__restrict__ lmap, const HaloCells &h, int64_t ystride,
+                                     int64_t polstride, int tid)
+{
+    if (h.own < 0)
+        return;
+    uint32_t touched = 0;
+    for (int g = tid; g < 3 * 64 * 8; g += NT) {
+        const int pl = g >> 9, r = (g >> 3) & 63, c0 = (g & 7) << 3;
+        touched |= __float_as_uint(lmap[h.own + pl * polstride + r * ystride + c0]);
+    }
+    asm volatile("" : : "v"(touched));
 }
 
 // Flush of the halo tile into the map. The cell's row 0 and its first HALO_X columns also
@@ -1234,6 +1258,7 @@ planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, con
             atomicMax(&s_max, mw);
     }
     t2m_clear<T2M_THREADS>(acc, true, threadIdx.x);
+    t2m_warm_cell<T2M_THREADS>(lmap, h, ystride, polstride, threadIdx.x);
     __syncthreads();
 
     float inv_scale;
@@ -1426,6 +1451,7 @@ batched_tod2map_kernel(float *__restrict__ lmap, const BatchTod *__restrict__ to
                 atomicMax(&s_max, mw);
         }
         t2m_clear<T2M_THREADS>(acc, true, opaque_tid());
+        t2m_warm_cell<T2M_THREADS>(lmap, h, ba.ystride, ba.polstride, opaque_tid());
         __syncthreads();
 
         float inv_scale;
