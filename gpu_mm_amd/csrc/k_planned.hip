@@ -750,6 +750,17 @@ __device__ inline void t2m_sample(unsigned long long *__restrict__ acc, float *_
     }
 }
 
+// sin and cos of x0 + d from s0 = sin x0, c0 = cos x0 for |d| <= 0.0185, where cos d = 1 - d^2/2
+// and sin d = d (1 - d^2/6) are exact to float rounding (the bound of the table's reduction).
+__device__ inline void angle_step(float s0, float c0, float d, float &sn, float &cs)
+{
+    const float z = d * d;
+    const float cd = fmaf(z, -0.5f, 1.0f);
+    const float sd = d * fmaf(z, -0.16666667f, 1.0f);
+    sn = fmaf(s0, cd, c0 * sd);
+    cs = fmaf(c0, cd, -(s0 * sd));
+}
+
 // Fixed-point tod2map of one 4-sample group (t already scaled). When the (iy0, ix0) corners of
 // the group's evaluated samples span at most two rows and two columns, the group's bilinear
 // footprint lies in a 3x3 pixel window: its contributions are summed per pixel in registers and
@@ -784,6 +795,14 @@ __device__ inline void t2m_quad_fixed(unsigned long long *__restrict__ acc, cons
     }
     if (ey < 0)
         return;
+    // sin and cos of 2 alpha: from the table for sample 0 and, when every lane's angles stay
+    // within |2 alpha_k - 2 alpha_0| <= 0.0185, for the other samples by angle addition from
+    // sample 0 (consecutive samples of one detector); otherwise from the table.
+    float s0, c0;
+    sincos2<Small>(tab, av[0], s0, c0);
+    const float x0 = 2.0f * av[0];
+    const float dmax = fmaxf(fmaxf(fabsf(2.0f * av[1] - x0), fabsf(2.0f * av[2] - x0)), fabsf(2.0f * av[3] - x0));
+    const bool near = Small && !__any(!(dmax <= 0.0185f));
     if (ey - by <= 1 && ex - bx <= 1) {
         float sum[3][3][3];
 #pragma unroll
@@ -793,27 +812,12 @@ __device__ inline void t2m_quad_fixed(unsigned long long *__restrict__ acc, cons
 #pragma unroll
                 for (int j = 0; j < 3; j++)
                     sum[p][i][j] = 0.0f;
-        // sin and cos of 2 alpha: from the table for sample 0 and, when every lane's angles
-        // stay within |2 alpha_k - 2 alpha_0| <= 0.0185 (the table's own reduction bound, where
-        // cos d = 1 - d^2/2 and sin d = d (1 - d^2/6) are exact to float rounding), for the other
-        // samples by angle addition from sample 0 (consecutive samples of one detector).
-        float s0, c0;
-        sincos2<Small>(tab, av[0], s0, c0);
-        const float x0 = 2.0f * av[0];
-        const float dmax = fmaxf(fmaxf(fabsf(2.0f * av[1] - x0), fabsf(2.0f * av[2] - x0)), fabsf(2.0f * av[3] - x0));
-        const bool near = Small && !__any(!(dmax <= 0.0185f));
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             float sn = s0, cs = c0;
             if (k > 0) {
-                if (near) {
-                    const float d = 2.0f * av[k] - x0;
-                    const float z = d * d;
-                    const float cd = fmaf(z, -0.5f, 1.0f);
-                    const float sd = d * fmaf(z, -0.16666667f, 1.0f);
-                    sn = fmaf(s0, cd, c0 * sd);
-                    cs = fmaf(c0, cd, -(s0 * sd));
-                }
+                if (near)
+                    angle_step(s0, c0, 2.0f * av[k] - x0, sn, cs);
                 else
                     sincos2<Small>(tab, av[k], sn, cs);
             }
@@ -874,8 +878,13 @@ __device__ inline void t2m_quad_fixed(unsigned long long *__restrict__ acc, cons
         for (int k = 0; k < 4; k++) {
             if (!own[k])
                 continue;
-            float sn, cs;
-            sincos2<Small>(tab, av[k], sn, cs);
+            float sn = s0, cs = c0;
+            if (k > 0) {
+                if (near)
+                    angle_step(s0, c0, 2.0f * av[k] - x0, sn, cs);
+                else
+                    sincos2<Small>(tab, av[k], sn, cs);
+            }
             const float ts = HasResp ? tv[k] * tr[k] : tv[k];
             const float ps = HasResp ? tv[k] * pr[k] : tv[k];
             deposit4<true>(acc, nullptr, ly[k] * HS + lx[k], dy[k], dx[k], ts, ps * cs, ps * sn);
