@@ -1118,7 +1118,8 @@ int gmm_test_plan_iterator(const uint64_t *plan_mt_cpu, int64_t nmt, int64_t nmt
             GMM_HIP(hipMemcpy(vis.data(), d_vis, size_t(nmt) * 4, hipMemcpyDeviceToHost));
             GMM_HIP(hipMemcpy(&e, d_err, 4, hipMemcpyDeviceToHost));
             if (e)
-                failure = "test_plan_iterator: a block visited an entry of another cell";
+                failure = (e & 1u) ? "test_plan_iterator: a block visited an entry of another cell"
+                                   : "test_plan_iterator: a wave visited an entry twice";
             for (int64_t i = 0; i < nmt && failure.empty(); i++)
                 if (vis[size_t(i)] != 1) {
                     std::ostringstream ss;

@@ -243,6 +243,17 @@ struct Chunk
 
 constexpr uint32_t CHUNK_SHARED = 1u;
 
+// Entries [wb, we) of a chunk handled by wave 'wave' of 'nw' in the planned operators: the chunk
+// split into nw contiguous, near-equal slices. Each wave then walks its slice in batches of 64
+// entries (one plan entry per lane, one vector load) and steps of 8 entries (lane 8*li + q takes
+// the 4-sample group q of entry li of the step).
+__host__ __device__ inline void wave_slice(const Chunk &ch, int wave, int nw, uint32_t &wb, uint32_t &we)
+{
+    const uint32_t n = ch.end - ch.begin;
+    wb = ch.begin + uint32_t((uint64_t(n) * uint32_t(wave)) / uint32_t(nw));
+    we = ch.begin + uint32_t((uint64_t(n) * uint32_t(wave + 1)) / uint32_t(nw));
+}
+
 // Batched operators (several TODs with their own plans, one local map). Device-resident tables.
 struct BatchTod
 {

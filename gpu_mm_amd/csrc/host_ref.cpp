@@ -148,7 +148,8 @@ double toy_pointing_fill(float *xp, int64_t nsamp, int64_t ny, int64_t nx, doubl
 // Brute-force planner of PointingPlanTester, src_lib/PointingPlanTester.cu:120-160: for every
 // 32-sample line, the distinct cells of the pixels (iy,ix), (iy,ix1), (iy+1,ix), (iy+1,ix1)
 // with ix1 = (ix < nx-1) ? ix+1 : 0, counted per planner block and emitted as
-// icell | (line << 20); the full list is then sorted as integers.
+// icell | (line << 20); the full list is then sorted as integers. iypix and ixpix hold nsamp
+// entries; a partial last line (nsamp % 32 != 0) has only its first nsamp % 32 samples.
 void plan_tester_host(const gmm_preplan &pp, const int32_t *iypix, const int32_t *ixpix,
                       uint32_t *nmt_cumsum, uint64_t *sorted_mt)
 {
@@ -162,7 +163,9 @@ void plan_tester_host(const gmm_preplan &pp, const int32_t *iypix, const int32_t
         for (int64_t s0 = b * spb; s0 < s1; s0 += 32) {
             uint32_t cells[128];
             int ncells = 0;
-            for (int64_t s = s0; s < s0 + 32; s++) {
+            // The last line of a TOD whose length is not a multiple of 32 is partial.
+            const int64_t se = std::min(nsamp, s0 + 32);
+            for (int64_t s = s0; s < se; s++) {
                 int iy = iypix[s], ix = ixpix[s];
                 int ix1 = (ix < nx - 1) ? (ix + 1) : 0;
                 const int py[4] = { iy, iy, iy + 1, iy + 1 };

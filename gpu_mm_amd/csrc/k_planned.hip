@@ -20,11 +20,13 @@
 //
 // tod2map accumulates in LDS with 64-bit integer atomics on fixed-point values: float LDS
 // atomics (ds_add_f32) run an order of magnitude slower than integer ones on this GPU. Deposits
-// are w*t*S with S = 2^(27-e) for a bound 2^(e+1) on |t|, so each converts exactly to int32 and
-// pixel sums are exact in the int64 tile; the flush converts each pixel once, float(sum)/S,
-// which is more accurate than float32 accumulation. The bound is estimated in the kernel (4x
-// the largest |t| in the first lines of the chunk); a chunk holding a larger value runs again
-// with its exact maximum, and one holding Inf/NaN runs again with float LDS atomics. The halo
+// are w*t*S with S = 2^(27-e) for a bound 2^(e+1) on the bulk of |t|, rounded to the nearest
+// integer (int32; int64 for the rare samples above the bound, so a glitch does not coarsen the
+// quantum of the cell's other samples), and pixel sums are exact in the int64 tile; the flush
+// converts each pixel once, float(sum)/S, which is more accurate than float32 accumulation. The
+// bound is estimated in the kernel (4x the largest |t| in the first lines of the chunk); a
+// chunk holding a value far beyond it runs again with a bound from its exact maximum, and one
+// holding Inf/NaN runs again with float LDS atomics (t2m_accumulate). The halo
 // row and columns, and the cell's own first row and first HALO_X columns (other blocks' halo
 // targets), are flushed to the map with atomics, the rest with a plain read-modify-write.
 // Reference: src_lib/tod2map.cu:33-154.
@@ -98,14 +100,6 @@ __device__ inline uint32_t xcd_chunk(uint32_t b, uint32_t n)
         return b;
     const uint32_t w = b % G;
     return (b - w) + (w % NUM_XCD) * XCD_RUN + w / NUM_XCD;
-}
-
-// Slice [wb, we) of the chunk's entries handled by wave 'wave' of 'nw'.
-__device__ inline void wave_slice(const Chunk &ch, int wave, int nw, uint32_t &wb, uint32_t &we)
-{
-    const uint32_t n = ch.end - ch.begin;
-    wb = ch.begin + uint32_t((uint64_t(n) * wave) / nw);
-    we = ch.begin + uint32_t((uint64_t(n) * (wave + 1)) / nw);
 }
 
 // Plan entry packed for broadcast: line index in bits [0,26), mflag in bit 31.
@@ -698,15 +692,35 @@ planned_map2tod_kernel(TT *__restrict__ tod, const float *__restrict__ lmap, con
 
 // ------------------------------------------------------------------------------------------
 
-// Fixed-point deposit of w*v (already scaled) into a 64-bit LDS accumulator.
-__device__ inline void dep_fixed(unsigned long long *__restrict__ a, float v)
+// Nearest integer of a float (ties away from -inf: floor(v + 0.5)) in one instruction; |v| < 2^31.
+// Rounding to nearest keeps the fixed-point deposits unbiased (truncation would pull every
+// deposit toward zero, which adds up on a TOD with an offset).
+__device__ inline int cvt_rpi(float v)
 {
-    const int iv = int(v);      // |v| < 2^28 by the choice of scale
-    atomicAdd(a, (unsigned long long)(long long)iv);
+    int r;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
 }
 
+// Fixed-point deposit of w*v (already scaled, |v| < 2^30) into a 64-bit LDS accumulator.
+__device__ inline void dep_fixed(unsigned long long *__restrict__ a, float v)
+{
+    atomicAdd(a, (unsigned long long)(long long)cvt_rpi(v));
+}
+
+// The same for a scaled value beyond the 32-bit range (an outlier sample, |v| < 2^52): rounded
+// to an integer in float (exact: a float of magnitude >= 2^23 is an integer already), then
+// converted to int64 exactly.
+__device__ inline void dep_wide(unsigned long long *__restrict__ a, float v)
+{
+    atomicAdd(a, (unsigned long long)(long long)rintf(v));
+}
+
+// Deposit forms: float LDS atomics (non-finite fallback), 32-bit fixed point, 64-bit fixed point.
+enum { DEP_FLOAT = 0, DEP_FIXED = 1, DEP_WIDE = 2 };
+
 // One sample's contributions to the four corners at halo-tile offset o: T, Q, U planes.
-template<bool Fixed>
+template<int Dep>
 __device__ inline void deposit4(unsigned long long *__restrict__ acc, float *__restrict__ facc, int o, float dy,
                                 float dx, float t, float tq, float tu)
 {
@@ -716,10 +730,15 @@ __device__ inline void deposit4(unsigned long long *__restrict__ acc, float *__r
 #pragma unroll
     for (int c = 0; c < 4; c++) {
         const int k = o + ofs[c];
-        if (Fixed) {
+        if (Dep == DEP_FIXED) {
             dep_fixed(acc + k, w[c] * t);
             dep_fixed(acc + HPLANE + k, w[c] * tq);
             dep_fixed(acc + 2 * HPLANE + k, w[c] * tu);
+        }
+        else if (Dep == DEP_WIDE) {
+            dep_wide(acc + k, w[c] * t);
+            dep_wide(acc + HPLANE + k, w[c] * tq);
+            dep_wide(acc + 2 * HPLANE + k, w[c] * tu);
         }
         else {
             atomicAdd(facc + k, w[c] * t);
@@ -732,7 +751,7 @@ __device__ inline void deposit4(unsigned long long *__restrict__ acc, float *__r
 // One sample of tod2map: locate; if the block evaluates it (its (iy0, ix0) corner lies in the
 // block's cell, or at local column < lxlim of the cell to the right), deposit all four corners
 // into the halo tile.
-template<bool Periodic, bool HasResp, bool Fixed, bool Small>
+template<bool Periodic, bool HasResp, int Dep, bool Small>
 __device__ inline void t2m_sample(unsigned long long *__restrict__ acc, float *__restrict__ facc,
                                   const float2 *__restrict__ tab, float y, float x, float alpha, float t, const Geom &g,
                                   int ybase, int xbase, unsigned lxlim, float tr, float pr, Checks &chk)
@@ -746,7 +765,7 @@ __device__ inline void t2m_sample(unsigned long long *__restrict__ acc, float *_
         sincos2<Small>(tab, alpha, sn, cs);
         const float ts = HasResp ? t * tr : t;
         const float ps = HasResp ? t * pr : t;
-        deposit4<Fixed>(acc, facc, ly * HS + lx, dy, dx, ts, ps * cs, ps * sn);
+        deposit4<Dep>(acc, facc, ly * HS + lx, dy, dx, ts, ps * cs, ps * sn);
     }
 }
 
@@ -766,7 +785,7 @@ __device__ inline void angle_step(float s0, float c0, float d, float &sn, float 
 // footprint lies in a 3x3 pixel window: its contributions are summed per pixel in registers and
 // added with one LDS atomic per touched pixel and plane (18 for a group that stays in one pixel
 // row, at most 27) instead of 12 per sample. A pixel's sum of at most four contributions, each
-// below 2^28 in magnitude, converts exactly to int32. Other groups deposit sample by sample.
+// below 2^28 in magnitude, rounds to an int32. Other groups deposit sample by sample.
 template<bool Periodic, bool HasResp, bool Small>
 __device__ inline void t2m_quad_fixed(unsigned long long *__restrict__ acc, const float2 *__restrict__ tab,
                                       const float4 &y4, const float4 &x4, const float4 &a4, const float4 &t4,
@@ -887,7 +906,7 @@ __device__ inline void t2m_quad_fixed(unsigned long long *__restrict__ acc, cons
             }
             const float ts = HasResp ? tv[k] * tr[k] : tv[k];
             const float ps = HasResp ? tv[k] * pr[k] : tv[k];
-            deposit4<true>(acc, nullptr, ly[k] * HS + lx[k], dy[k], dx[k], ts, ps * cs, ps * sn);
+            deposit4<DEP_FIXED>(acc, nullptr, ly[k] * HS + lx[k], dy[k], dx[k], ts, ps * cs, ps * sn);
         }
     }
 }
@@ -928,15 +947,18 @@ __device__ inline float t2m_probe(const TT *__restrict__ tod, const uint64_t *__
 }
 
 // tod2map of plan entries [wb, we) of one TOD by the calling wave into the block's halo tile:
-// fixed point with scale 'scale' when 'finite', else float atomics. Tracks per lane the largest
-// |t| deposited (mrun) and a sum that is NaN iff some deposited t is NaN (tsum).
+// fixed point with scale 'scale' when 'finite', else float atomics. In fixed point a group whose
+// largest |t| (times the gains) is below 'limit' takes the 32-bit path, any other group (an
+// outlier) deposits sample by sample in 64-bit fixed point and counts in nwide. Tracks per lane
+// the largest |t| of its outlier groups (mrun) and a sum that is NaN iff some deposited t is NaN
+// (tsum).
 template<bool Periodic, bool Vec, bool HasResp, typename TT>
 __device__ inline void t2m_entries(unsigned long long *__restrict__ acc, float *__restrict__ facc,
                                    const TT *__restrict__ tod, const float *__restrict__ xp,
                                    const uint64_t *__restrict__ plan_mt, const uint16_t *__restrict__ qmask,
                                    int64_t nsamp, uint32_t wb, uint32_t we, const float2 *__restrict__ sct,
                                    const Geom &g, int ybase, int xbase, const Gains &gn, bool finite, float scale,
-                                   float &mrun, float &tsum, Checks &chk, uint32_t &err)
+                                   float limit, float &mrun, float &tsum, uint32_t &nwide, Checks &chk, uint32_t &err)
 {
     constexpr bool Q16 = !std::is_same<TT, float>::value;
     using IO = TodIO<TT, Vec>;
@@ -985,29 +1007,41 @@ __device__ inline void t2m_entries(unsigned long long *__restrict__ acc, float *
             }
             if (finite) {
                 // NaN makes fmaxf skip the value: count it through the sum instead.
+                float tmax;
                 if (HasResp) {
-                    mrun = fmaxf(mrun, fmaxf(fabsf(t4.x * tr[0]), fabsf(t4.x * pr[0])));
-                    mrun = fmaxf(mrun, fmaxf(fabsf(t4.y * tr[1]), fabsf(t4.y * pr[1])));
-                    mrun = fmaxf(mrun, fmaxf(fabsf(t4.z * tr[2]), fabsf(t4.z * pr[2])));
-                    mrun = fmaxf(mrun, fmaxf(fabsf(t4.w * tr[3]), fabsf(t4.w * pr[3])));
+                    tmax = fmaxf(fmaxf(fabsf(t4.x * tr[0]), fabsf(t4.x * pr[0])),
+                                 fmaxf(fabsf(t4.y * tr[1]), fabsf(t4.y * pr[1])));
+                    tmax = fmaxf(tmax, fmaxf(fmaxf(fabsf(t4.z * tr[2]), fabsf(t4.z * pr[2])),
+                                             fmaxf(fabsf(t4.w * tr[3]), fabsf(t4.w * pr[3]))));
                 }
                 else
-                    mrun = fmaxf(mrun, fmaxf(fmaxf(fabsf(t4.x), fabsf(t4.y)), fmaxf(fabsf(t4.z), fabsf(t4.w))));
+                    tmax = fmaxf(fmaxf(fabsf(t4.x), fabsf(t4.y)), fmaxf(fabsf(t4.z), fabsf(t4.w)));
                 tsum += (t4.x + t4.y) + (t4.z + t4.w);
                 t4.x *= scale;
                 t4.y *= scale;
                 t4.z *= scale;
                 t4.w *= scale;
-                if (quad_small_angles(a4))
-                    t2m_quad_fixed<Periodic, HasResp, true>(acc, sct, y4, x4, a4, t4, g, ybase, xbase, lxlim, tr, pr, chk);
-                else
-                    t2m_quad_fixed<Periodic, HasResp, false>(acc, sct, y4, x4, a4, t4, g, ybase, xbase, lxlim, tr, pr, chk);
+                if (tmax < limit) {
+                    if (quad_small_angles(a4))
+                        t2m_quad_fixed<Periodic, HasResp, true>(acc, sct, y4, x4, a4, t4, g, ybase, xbase, lxlim, tr, pr, chk);
+                    else
+                        t2m_quad_fixed<Periodic, HasResp, false>(acc, sct, y4, x4, a4, t4, g, ybase, xbase, lxlim, tr, pr, chk);
+                }
+                else {
+                    // Only an outlier group can exceed the 64-bit range: mrun is tracked here.
+                    mrun = fmaxf(mrun, tmax);
+                    nwide++;
+                    t2m_sample<Periodic, HasResp, DEP_WIDE, false>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, lxlim, tr[0], pr[0], chk);
+                    t2m_sample<Periodic, HasResp, DEP_WIDE, false>(acc, facc, sct, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, lxlim, tr[1], pr[1], chk);
+                    t2m_sample<Periodic, HasResp, DEP_WIDE, false>(acc, facc, sct, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, lxlim, tr[2], pr[2], chk);
+                    t2m_sample<Periodic, HasResp, DEP_WIDE, false>(acc, facc, sct, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, lxlim, tr[3], pr[3], chk);
+                }
             }
             else {
-                t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, lxlim, tr[0], pr[0], chk);
-                t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, lxlim, tr[1], pr[1], chk);
-                t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, lxlim, tr[2], pr[2], chk);
-                t2m_sample<Periodic, HasResp, false, false>(acc, facc, sct, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, lxlim, tr[3], pr[3], chk);
+                t2m_sample<Periodic, HasResp, DEP_FLOAT, false>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, lxlim, tr[0], pr[0], chk);
+                t2m_sample<Periodic, HasResp, DEP_FLOAT, false>(acc, facc, sct, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, lxlim, tr[1], pr[1], chk);
+                t2m_sample<Periodic, HasResp, DEP_FLOAT, false>(acc, facc, sct, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, lxlim, tr[2], pr[2], chk);
+                t2m_sample<Periodic, HasResp, DEP_FLOAT, false>(acc, facc, sct, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, lxlim, tr[3], pr[3], chk);
             }
         }
     }
@@ -1171,23 +1205,44 @@ __device__ inline void t2m_flush(float *__restrict__ lmap, const unsigned long l
     }
 }
 
-// Deposits of one block's work unit into its halo tile, in fixed point with redo rounds.
-// Before the call the tile is clear (fixed-point form), *s_max holds the float bits of the
-// block's probe maximum, s_redo[0..2] are zero, and the block has synchronized.
-// run(finite, scale, mrun, tsum, chk) deposits the calling wave's samples (t2m_entries()).
-// Deposits are w * t * S with S = 2^(27 - e) and |t| < 2^(e+1), so each converts exactly to
-// int32 (|.| < 2^28) and the tile sums in int64. The exponent e comes from an estimate: 4 x the
-// probe maximum. Each lane tracks the largest |t| it deposits; if any exceeds the estimate's
-// range, or is not finite, the block discards its tile and runs again, with the exact maximum
-// (which cannot trigger another round), or with float atomics when that maximum is not finite:
-// at most three rounds, each with its own redo word. Returns whether the tile is fixed point
-// and its inverse scale; ends synchronized.
+// Block-shared words of t2m_accumulate(): the probe maximum (float bits), and per round a redo
+// word (largest offending |t|, as float bits) and a count of outlier groups.
+enum { W_MAX = 0, W_REDO = 1, W_WIDE = 4, T2M_WORDS = 7 };
+
+// Outliers: the 64-bit path takes scaled values up to 2^WIDE_BITS times the 32-bit range (|v| <
+// 2^52), so a pixel's int64 sum stays exact for the first 2^11 deposits at the top of that range.
+constexpr int WIDE_BITS = 24;
+
+__device__ inline uint32_t wave_sum(uint32_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v += uint32_t(__shfl_xor(v, off, WAVE));
+    return v;
+}
+
+// Deposits of one block's work unit (nentries plan entries) into its halo tile, in fixed point.
+// Before the call the tile is clear (fixed-point form), words[W_MAX] holds the float bits of the
+// block's probe maximum, the other words are zero, and the block has synchronized.
+// run(finite, scale, limit, mrun, tsum, nwide, chk) deposits the calling wave's samples
+// (t2m_entries()). With a bound 2^(e+1) on the bulk of |t|, deposits are w * t * S with
+// S = 2^(27 - e): a group below the bound converts each (3x3-window) pixel sum to int32 (|.| <
+// 2^30); a group above it (an outlier such as a glitch) deposits each contribution as int64,
+// exactly, without coarsening the quantum of the other samples of the cell. The tile sums in
+// int64; every conversion rounds to nearest. The bound starts from an estimate, 4 x the probe
+// maximum. A value beyond the 64-bit range (2^WIDE_BITS times the bound) or a non-finite one
+// makes the block discard its tile and run again: with float atomics when the exact maximum M
+// is not finite; else with the bound at M when at least one group in 8 was an outlier (the
+// estimate was far too low, e.g. zeros at the probed lines), or at 2^-WIDE_BITS M otherwise
+// (rare huge outliers: the bulk keeps a fine quantum). Either bound puts M inside the 64-bit
+// range, so the second round is the last. Each round has its own words. Returns whether the
+// tile is fixed point and its inverse scale; ends synchronized.
 template<int NT, typename Run>
-__device__ inline bool t2m_accumulate(unsigned long long *__restrict__ acc, const uint32_t *s_max, uint32_t *s_redo,
+__device__ inline bool t2m_accumulate(unsigned long long *__restrict__ acc, uint32_t *words, uint32_t nentries,
                                       const Geom &g, Run &&run, float &inv_scale, uint32_t &err, int tid)
 {
     const int lane = threadIdx.x & 63;
-    uint32_t mbits = *s_max;
+    uint32_t mbits = words[W_MAX];
     mbits = (mbits < FINITE_LIMIT) ? __float_as_uint(4.0f * __uint_as_float(mbits)) : mbits;
     bool finite = (mbits < FINITE_LIMIT);
 
@@ -1195,26 +1250,36 @@ __device__ inline bool t2m_accumulate(unsigned long long *__restrict__ acc, cons
         const int e = int(mbits >> 23) - 127;            // binary exponent of the bound
         const float scale = __uint_as_float(uint32_t(t2m_scale_exp(mbits) + 127) << 23);
         const float limit = __uint_as_float(uint32_t(e + 128) << 23);   // 2^(e+1)
-        float mrun = 0.0f;                               // largest |t| deposited by this lane
+        const uint32_t wide_limit = uint32_t(min(e + 128 + WIDE_BITS, 255)) << 23;
+        float mrun = 0.0f;                               // largest |t| of this lane's outlier groups
         float tsum = 0.0f;                               // NaN iff some deposited t is NaN
+        uint32_t nwide = 0;                              // outlier groups of this lane
         Checks chk;
-        run(finite, scale, mrun, tsum, chk);
+        run(finite, scale, limit, mrun, tsum, nwide, chk);
         err |= chk.bits(g);
         if (!finite)
             break;
         const bool lane_nan = !(tsum == tsum);
-        uint32_t *redo_word = s_redo + min(pass, 2);
-        // Any value beyond the estimate's range or non-finite: redo.
+        const int r = min(pass, 2);
         const uint32_t mw = wave_max(lane_nan ? FINITE_LIMIT : __float_as_uint(mrun));
-        if (lane == 0 && mw >= __float_as_uint(limit))
-            atomicMax(redo_word, mw);
+        const uint32_t nw = wave_sum(nwide);
+        if (lane == 0) {
+            if (mw >= wide_limit)
+                atomicMax(words + W_REDO + r, mw);
+            if (nw)
+                atomicAdd(words + W_WIDE + r, nw);
+        }
         __syncthreads();
-        const uint32_t redo = *redo_word;
+        const uint32_t redo = words[W_REDO + r];
         if (redo == 0)
             break;
-        // Redo with the exact maximum (now within range), or in floating point.
-        mbits = redo;
-        finite = (mbits < FINITE_LIMIT);
+        finite = (redo < FINITE_LIMIT);
+        if (!finite)
+            mbits = redo;
+        else if (words[W_WIDE + r] >= nentries)
+            mbits = redo;
+        else
+            mbits = uint32_t(max(int(redo >> 23) - 127 - WIDE_BITS, -126) + 127) << 23;
         t2m_clear<NT>(acc, finite, tid);
         __syncthreads();
     }
@@ -1233,7 +1298,7 @@ planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, con
 {
     __shared__ __align__(16) unsigned long long acc[3 * HPLANE];       // 116 KiB
     __shared__ float2 sct[SCT];
-    __shared__ uint32_t s_max, s_redo[3];
+    __shared__ uint32_t s_words[T2M_WORDS];
     float *facc = reinterpret_cast<float *>(acc);         // float tile for the non-finite fallback
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1252,19 +1317,15 @@ planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, con
 
     // The fixed-point bound is estimated in the kernel from the first lines of every wave's
     // slice (t2m_accumulate() describes the scheme): no pass over the TOD is needed first.
-    if (threadIdx.x < 4) {
-        if (threadIdx.x == 0)
-            s_max = 0;
-        else
-            s_redo[threadIdx.x - 1] = 0;
-    }
+    if (threadIdx.x < T2M_WORDS)
+        s_words[threadIdx.x] = 0;
     fill_sincos_table(sct, T2M_THREADS);
     __syncthreads();
     {
         const float m = t2m_probe<Vec, HasResp, TT>(tod, plan_mt, nsamp, wb, we, gn);
         const uint32_t mw = wave_max(__float_as_uint(m));
         if (lane == 0)
-            atomicMax(&s_max, mw);
+            atomicMax(&s_words[W_MAX], mw);
     }
     t2m_clear<T2M_THREADS>(acc, true, threadIdx.x);
     t2m_warm_cell<T2M_THREADS>(lmap, h, ystride, polstride, threadIdx.x);
@@ -1272,10 +1333,10 @@ planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, con
 
     float inv_scale;
     const bool finite = t2m_accumulate<T2M_THREADS>(
-        acc, &s_max, s_redo, g,
-        [&](bool fin, float scale, float &mrun, float &tsum, Checks &chk) {
+        acc, s_words, ch.end - ch.begin, g,
+        [&](bool fin, float scale, float limit, float &mrun, float &tsum, uint32_t &nwide, Checks &chk) {
             t2m_entries<Periodic, Vec, HasResp, TT>(acc, facc, tod, xp, plan_mt, qmask, nsamp, wb, we, sct, g, ybase,
-                                                    xbase, gn, fin, scale, mrun, tsum, chk, err);
+                                                    xbase, gn, fin, scale, limit, mrun, tsum, nwide, chk, err);
         },
         inv_scale, err, threadIdx.x);
     t2m_flush<T2M_THREADS>(lmap, acc, facc, h, (ch.flags & CHUNK_SHARED) != 0, finite, inv_scale, ystride,
@@ -1416,7 +1477,7 @@ batched_tod2map_kernel(float *__restrict__ lmap, const BatchTod *__restrict__ to
 {
     __shared__ __align__(16) unsigned long long acc[3 * HPLANE];       // 116 KiB
     __shared__ float2 sct[SCT];
-    __shared__ uint32_t s_max, s_redo[3], s_item;
+    __shared__ uint32_t s_words[T2M_WORDS], s_item;
     float *facc = reinterpret_cast<float *>(acc);
 
     fill_sincos_table(sct, T2M_THREADS);
@@ -1426,10 +1487,8 @@ batched_tod2map_kernel(float *__restrict__ lmap, const BatchTod *__restrict__ to
     uint32_t err = 0;
 
     for (;;) {
-        if (threadIdx.x == 1)
-            s_max = 0;
-        else if (threadIdx.x >= 2 && threadIdx.x < 5)
-            s_redo[threadIdx.x - 2] = 0;
+        if (threadIdx.x >= 1 && threadIdx.x <= T2M_WORDS)   // thread 0 may be writing s_item
+            s_words[threadIdx.x - 1] = 0;
         if (!ba.persistent)
             __syncthreads();
         const uint32_t item = batch_next_item(ba, &s_item);
@@ -1457,7 +1516,7 @@ batched_tod2map_kernel(float *__restrict__ lmap, const BatchTod *__restrict__ to
             });
             const uint32_t mw = wave_max(__float_as_uint(m));
             if (lane == 0)
-                atomicMax(&s_max, mw);
+                atomicMax(&s_words[W_MAX], mw);
         }
         t2m_clear<T2M_THREADS>(acc, true, opaque_tid());
         t2m_warm_cell<T2M_THREADS>(lmap, h, ba.ystride, ba.polstride, opaque_tid());
@@ -1465,13 +1524,14 @@ batched_tod2map_kernel(float *__restrict__ lmap, const BatchTod *__restrict__ to
 
         float inv_scale;
         const bool finite = t2m_accumulate<T2M_THREADS>(
-            acc, &s_max, s_redo, g,
-            [&](bool fin, float scale, float &mrun, float &tsum, Checks &chk) {
+            acc, s_words, it.nent, g,
+            [&](bool fin, float scale, float limit, float &mrun, float &tsum, uint32_t &nwide, Checks &chk) {
                 batch_wave_segments(segs, it, wb, we, [&](const BatchSeg &sg, uint32_t b, uint32_t e) {
                     const BatchTod td = tods[sg.tod];
                     t2m_entries<Periodic, true, HasResp, TT>(acc, facc, static_cast<const TT *>(td.tod), td.xp,
                                                            td.plan_mt, td.qmask, td.nsamp, b, e, sct, g, ybase, xbase,
-                                                           batch_gains(td), fin, scale, mrun, tsum, chk, err);
+                                                           batch_gains(td), fin, scale, limit, mrun, tsum, nwide, chk,
+                                                           err);
                 });
             },
             inv_scale, err, opaque_tid());

@@ -672,22 +672,36 @@ __global__ void quantize_kernel(const float *__restrict__ xp, int64_t nsamp, int
         atomicOr(err_word, err);
 }
 
-// Walks the work list the way the operators do (entry pairs per wave, waves strided) and
-// counts visits; a visit to an entry of another cell sets an error bit.
+// Walks the work list exactly as the planned operators do: each wave takes its wave_slice() of
+// the chunk, in batches of 64 entries (one per lane, one load) and steps of 8 entries (lane
+// 8*li + q on entry li of the step, with the entry broadcast from the batch by a shuffle), and
+// counts the visits of each entry (lane q == 0); a visit to an entry of another cell sets bit 0
+// of the error word, an entry reached twice within one wave bit 1.
 __global__ void iterator_test_kernel(const uint64_t *__restrict__ plan_mt, const Chunk *__restrict__ chunks,
                                      uint32_t *__restrict__ visits, uint32_t *__restrict__ err_word)
 {
     const int lane = lane_id(), wave = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+    const int li = lane >> 3, q = lane & 7;
     const Chunk ch = chunks[blockIdx.x];
-    uint32_t err = 0;
-    for (uint32_t e = ch.begin + 2 * wave; e < ch.end; e += 2 * nw) {
-        uint32_t my = e + (lane >> 5);
-        if (my < ch.end) {
-            uint64_t mt = plan_mt[my];
-            if ((uint32_t(mt) & CELL_MASK) != ch.icell)
+    uint32_t wb, we;
+    wave_slice(ch, wave, nw, wb, we);
+    uint32_t err = 0, last = 0xffffffffu;
+    for (uint32_t base = wb; base < we; base += 64) {
+        const uint32_t cnt = min(64u, we - base);
+        const uint32_t key = (uint32_t(lane) < cnt) ? (uint32_t(plan_mt[base + lane]) & CELL_MASK) : 0u;
+        for (uint32_t j = 0; j < cnt; j += 8) {
+            const uint32_t idx = j + li;
+            const uint32_t k = __shfl(key, int(min(idx, 63u)), WAVE);
+            if (idx >= cnt)
+                continue;
+            const uint32_t e = base + idx;
+            if (k != ch.icell)
                 err |= 1u;
-            if ((lane & 31) == 0)
-                atomicAdd(visits + my, 1u);
+            if (last != 0xffffffffu && e <= last)
+                err |= 2u;
+            last = e;
+            if (q == 0)
+                atomicAdd(visits + e, 1u);
         }
     }
     err = wave_reduce_or(err);

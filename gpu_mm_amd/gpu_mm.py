@@ -43,8 +43,9 @@ from . import _native
 _ext = _native.ext()
 
 # The planned and unplanned operators check the GPU error flags synchronously after every
-# call (as the reference does). set_errflag_check(False) makes calls asynchronous; errors are
-# then accumulated on the plan and raised by PointingPlan.check_errflags().
+# call (as the reference does). set_errflag_check(False) makes planned calls asynchronous;
+# errors are then accumulated on the plan and raised by PointingPlan.check_errflags().
+# Plan-free calls (plan=None) always check synchronously.
 _errflag_check = True
 
 mm_dtype = np.float32
@@ -163,9 +164,10 @@ def map2tod(tod, local_map, xpointing, plan, partial_pixelization=False, respons
             _ext.response_map2tod(tod, larr, xpointing, response, lpix, plan, partial_pixelization, debug,
                                   accum, check_errors=_errflag_check, stream=_stream())
     elif plan is None:
-        errflags = torch.zeros(_UNPLANNED_MAX_BLOCKS, dtype=torch.uint32, device=_device())
-        _ext.unplanned_map2tod(tod, larr, xpointing, lpix, errflags, partial_pixelization, accum,
-                               check_errors=_errflag_check, stream=_stream())
+        # Plan-free calls always check their error flags synchronously: there is no plan to
+        # accumulate them on.
+        _ext.unplanned_map2tod(tod, larr, xpointing, lpix, _unplanned_errflags(), partial_pixelization, accum,
+                               check_errors=True, stream=_stream())
     elif isinstance(plan, str) and plan == 'reference':
         _ext.reference_map2tod(tod, larr, xpointing, lpix, partial_pixelization, accum)
     else:
@@ -190,8 +192,7 @@ def tod2map(local_map, tod, xpointing, plan, partial_pixelization=False, respons
                                + f" but DynamicMap.have_cell_mask={local_map.have_cell_mask}"
                                + " (expected these to be equal)")
         local_map.expand(plan)
-        lpix = local_map._unstable_pixelization
-        larr = local_map._unstable_arr
+        lpix, larr = local_map._target()
     elif isinstance(local_map, LocalMap):
         lpix = local_map.pixelization
         larr = local_map.arr
@@ -213,17 +214,21 @@ def tod2map(local_map, tod, xpointing, plan, partial_pixelization=False, respons
             _ext.response_tod2map(larr, tod, xpointing, response, lpix, plan, partial_pixelization, debug,
                                   check_errors=_errflag_check, stream=_stream())
     elif plan is None:
-        errflags = torch.zeros(_UNPLANNED_MAX_BLOCKS, dtype=torch.uint32, device=_device())
-        _ext.unplanned_tod2map(larr, tod, xpointing, lpix, errflags, partial_pixelization,
-                               check_errors=_errflag_check, stream=_stream())
+        _ext.unplanned_tod2map(larr, tod, xpointing, lpix, _unplanned_errflags(), partial_pixelization,
+                               check_errors=True, stream=_stream())
     elif isinstance(plan, str) and plan == 'reference':
         _ext.reference_tod2map(larr, tod, xpointing, lpix, partial_pixelization)
     else:
         raise RuntimeError(f"Bad 'plan' argument to tod2map(): {plan}")
 
 
-# Block budget of the plan-free kernels (gpu_mm.py:306 uses 2048 as well).
+# Block budget of the plan-free kernels (gpu_mm.py:306 uses 2048 as well). Every block of a
+# plan-free launch writes its own error-flag word, so the buffer needs no clearing.
 _UNPLANNED_MAX_BLOCKS = 2048
+
+
+def _unplanned_errflags():
+    return torch.empty(_UNPLANNED_MAX_BLOCKS, dtype=torch.uint32, device=_device())
 
 
 ####################################################################################################
@@ -381,128 +386,153 @@ class LocalMap:
 
 class DynamicMap:
     """
-    A local map that starts with no cells and gains cells as tod2map() needs them; finalize()
-    turns it into a LocalMap. Layout: float32 local_map[ncells][3][64][64] (ystride 64,
-    polstride 4096). cell_mask (bool, shape (nycells, nxcells)) restricts the cells that may be
-    added ("targeted" cells); tod2map() then needs partial_pixelization=True.
+    Local map whose cells are created on demand by tod2map(): it starts empty, each
+    tod2map(dmap, tod, xpointing, plan) call adds (zero-filled) the cells the plan touches, and
+    finalize() hands the result over as a LocalMap with layout float32[ncells][3][64][64]
+    (ystride 64, polstride 4096; cell i at offset 3*64*64*i of the array).
 
-    New cells get indices in increasing cell-key order within each expansion, so the layout is
-    reproducible (the reference's order is nondeterministic). Only tod2map() accepts a
-    DynamicMap, as in the reference.
+    cell_mask (bool, shape (nycells, nxcells)) limits the cells that may be created to a target
+    region; tod2map() then needs partial_pixelization=True, and samples outside the region
+    deposit nothing.
 
-    Reference: gpu_mm/gpu_mm.py:586-806; src_lib/expand_dynamic_map.cu.
+    How it is held here:
+      - the cell store: one float32 GPU tensor of shape (capacity, 3, 64, 64), whose first
+        ncells_curr cells are the map; when a plan needs more cells the capacity doubles (or
+        jumps to what is needed), and the filled cells move over;
+      - the cell table: an int64 GPU array (nycells, nxcells) inside a LocalPixelization: the
+        store index times 3*64*64 for present cells, -1 for cells that may still be created,
+        -2 for cells outside the target region. It is updated on the GPU (cell creation is a
+        device kernel, numbering new cells in cell-key order, so layouts are reproducible); its
+        host copy is brought up to date only when the layout is read (permute, finalize);
+      - a lifecycle: "open" (cells may be added) -> "fixed" (after permute: the layout no
+        longer changes) -> "done" (after finalize: the map belongs to the returned LocalMap).
+
+    Only tod2map() accepts a DynamicMap. API (constructor, expand, permute, finalize,
+    randomly_permute, have_cell_mask): gpu_mm/gpu_mm.py:586-806; cell creation:
+    src_lib/expand_dynamic_map.cu.
     """
 
+    _CELL_FLOATS = 3 * 64 * 64
+
     def __init__(self, nypix_global, nxpix_global, dtype, cell_mask=None, periodic_xcoord=True, initial_ncells=1024):
-        assert 0 < nxpix_global <= 64 * 1024
-        assert 0 < nypix_global <= 64 * 1024
-        assert dtype in (np.float32, torch.float32, 'float32'), "DynamicMap: only float32 is supported"
-        assert initial_ncells > 0
+        if not (0 < nypix_global <= 64 * 1024 and 0 < nxpix_global <= 64 * 1024):
+            raise RuntimeError(f"DynamicMap: expected 0 < nypix_global, nxpix_global <= 65536, got"
+                               f" {(nypix_global, nxpix_global)}")
+        if dtype not in (np.float32, torch.float32, 'float32'):
+            raise RuntimeError(f"DynamicMap: only float32 maps are supported (got {dtype})")
+        if initial_ncells <= 0:
+            raise RuntimeError(f"DynamicMap: expected initial_ncells > 0, got {initial_ncells}")
 
-        self.nypix_global = int(nypix_global)
-        self.nxpix_global = int(nxpix_global)
+        self.nypix_global, self.nxpix_global = int(nypix_global), int(nxpix_global)
         self.periodic_xcoord = bool(periodic_xcoord)
-        self.have_cell_mask = cell_mask is not None
-        self.ncells_allocated = int(initial_ncells)
-        self.reallocation_factor = 1.5
         self.dtype = np.float32
+        shape = ((self.nypix_global + 63) // 64, (self.nxpix_global + 63) // 64)
 
-        nycells = (self.nypix_global + 63) // 64
-        nxcells = (self.nxpix_global + 63) // 64
         if cell_mask is None:
-            cell_offsets = np.full((nycells, nxcells), -1, dtype=np.int64)
+            self._mask = None
+            table = np.full(shape, -1, dtype=np.int64)
         else:
-            cell_mask = np.asarray(cell_mask)
-            assert cell_mask.ndim == 2 and cell_mask.dtype == bool
-            assert cell_mask.shape == (nycells, nxcells)
-            cell_offsets = np.where(cell_mask, -1, -2).astype(np.int64)
+            self._mask = np.asarray(cell_mask)
+            if self._mask.dtype != bool or self._mask.shape != shape:
+                raise RuntimeError(f"DynamicMap: cell_mask must be a bool array of shape {shape}")
+            table = np.where(self._mask, -1, -2).astype(np.int64)
 
-        # Between calls the CPU copy of the offsets is stale (the GPU copy is authoritative).
-        self._unstable_pixelization = LocalPixelization(self.nypix_global, self.nxpix_global, cell_offsets,
-                                                        ystride=64, polstride=64 * 64,
-                                                        periodic_xcoord=self.periodic_xcoord)
-        self._padded_arr = torch.zeros(self.ncells_allocated * 3 * 64 * 64, dtype=torch.float32, device=_device())
-        self._unstable_arr = self._padded_arr[:0]
-        self.global_ncells = torch.zeros(1, dtype=torch.uint32, device=_device())
-        self.ncells_curr = 0
-        self.pixelization_is_finalized = False
-        self.map_is_finalized = False
+        self._table = LocalPixelization(self.nypix_global, self.nxpix_global, table, ystride=64,
+                                        polstride=64 * 64, periodic_xcoord=self.periodic_xcoord)
+        self._store = torch.zeros((int(initial_ncells), 3, 64, 64), dtype=torch.float32, device=_device())
+        self._count = torch.zeros(1, dtype=torch.uint32, device=_device())   # cells in use (device side)
+        self._ncells = 0
+        self._state = "open"
+
+    # Read-only state (names as in the reference).
+    have_cell_mask = property(lambda self: self._mask is not None)
+    ncells_curr = property(lambda self: self._ncells)
+    ncells_allocated = property(lambda self: 0 if self._store is None else int(self._store.shape[0]))
+    pixelization_is_finalized = property(lambda self: self._state != "open")
+    map_is_finalized = property(lambda self: self._state == "done")
+
+    def _target(self):
+        """(LocalPixelization, flat float32 map) for tod2map(); valid until the next expansion."""
+        return self._table, self._store[:self._ncells].view(-1)
+
+    def _reserve(self, ncells):
+        cap = int(self._store.shape[0])
+        if ncells <= cap:
+            return
+        grown = torch.zeros((max(ncells, 2 * cap), 3, 64, 64), dtype=torch.float32, device=self._store.device)
+        grown[:self._ncells].copy_(self._store[:self._ncells])
+        self._store = grown
 
     def expand(self, plan):
-        """Adds the cells a plan touches (zero-filled), reallocating the array when needed."""
-        if self.pixelization_is_finalized:
+        """Creates (zero-filled) every cell the plan touches that the map does not hold yet."""
+        if self._state != "open":
             raise RuntimeError("Can't currently call tod2map() [or DynamicMap.expand()]"
                                + " after DynamicMap.permute() or DynamicMap.finalize().")
-        assert isinstance(plan, PointingPlan)
-        assert plan.nypix_global == self.nypix_global
-        assert plan.nxpix_global == self.nxpix_global
-        assert plan.periodic_xcoord == self.periodic_xcoord
+        if not isinstance(plan, PointingPlan):
+            raise RuntimeError(f"DynamicMap.expand(): expected a PointingPlan, got {plan!r}")
+        if (plan.nypix_global, plan.nxpix_global, plan.periodic_xcoord) != \
+                (self.nypix_global, self.nxpix_global, self.periodic_xcoord):
+            raise RuntimeError("DynamicMap.expand(): the plan's global pixelization (nypix_global, nxpix_global,"
+                               " periodic_xcoord) differs from the map's")
+        n = int(_ext.expand_dynamic_map2(self._count, self._table, plan, stream=_stream()))
+        self._reserve(n)
+        self._ncells = n
+        self._table.npix = n * 64 * 64
 
-        self.ncells_curr = int(_ext.expand_dynamic_map2(self.global_ncells, self._unstable_pixelization, plan,
-                                                         stream=_stream()))
-        if self.ncells_curr > self.ncells_allocated:
-            self.ncells_allocated = max(int(self.reallocation_factor * self.ncells_allocated), self.ncells_curr)
-            new = torch.zeros(self.ncells_allocated * 3 * 64 * 64, dtype=torch.float32, device=_device())
-            new[:self._unstable_arr.numel()] = self._unstable_arr
-            self._padded_arr = new
-        self._unstable_pixelization.npix = self.ncells_curr * 64 * 64
-        self._unstable_arr = self._padded_arr[:self.ncells_curr * 3 * 64 * 64]
+    def _layout(self):
+        """Host copy of the cell table, refreshed from the GPU while the layout is open."""
+        if self._state == "open":
+            self._table.copy_gpu_offsets_to_cpu()
+        return self._table.cell_offsets_cpu
 
     def permute(self, iycells, ixcells):
         """
-        Reorders the cells: cell i of the new layout is (iycells[i], ixcells[i]); the arrays (int,
-        length ncells_curr, on the CPU) must list exactly the present cells. Finalizes the
-        pixelization (no more expansion).
+        New cell order: cell i of the result is (iycells[i], ixcells[i]). The two integer arrays
+        (host, length ncells_curr) must name every present cell once. Fixes the layout: no cells
+        can be added afterwards.
         """
-        if self.pixelization_is_finalized:
+        if self._state != "open":
             raise RuntimeError("Can't call DynamicMap.permute() after DynamicMap.permute() or DynamicMap.finalize()")
-        lp = self._unstable_pixelization
-        lp.copy_gpu_offsets_to_cpu()
-        assert lp.npix == self.ncells_curr * 64 * 64
-        old = np.copy(lp.cell_offsets_cpu)
-        iycells, ixcells = np.asarray(iycells), np.asarray(ixcells)
-        assert iycells.shape == ixcells.shape == (self.ncells_curr,)
-        assert np.issubdtype(iycells.dtype, np.integer) and np.issubdtype(ixcells.dtype, np.integer)
-        if self.ncells_curr > 0:
-            assert 0 <= iycells.min() and iycells.max() < old.shape[0]
-            assert 0 <= ixcells.min() and ixcells.max() < old.shape[1]
-        new = np.copy(old)
-        new[iycells, ixcells] = np.arange(self.ncells_curr, dtype=np.int64) * (3 * 64 * 64)
-        if not np.array_equal(old >= 0, new >= 0) or len(set(zip(iycells.tolist(), ixcells.tolist()))) != self.ncells_curr:
+        table = self._layout()
+        n = self._ncells
+        iy = np.asarray(iycells)
+        ix = np.asarray(ixcells)
+        if iy.shape != (n,) or ix.shape != (n,) or not (np.issubdtype(iy.dtype, np.integer)
+                                                      and np.issubdtype(ix.dtype, np.integer)):
+            raise RuntimeError(f"DynamicMap.permute(): expected two integer arrays of length {n}")
+        inside = np.all((iy >= 0) & (iy < table.shape[0]) & (ix >= 0) & (ix < table.shape[1]))
+        src = table[iy, ix] if (inside and n > 0) else np.zeros(0, dtype=np.int64)
+        if not inside or np.any(src < 0) or np.unique(src).size != n:
             raise RuntimeError("DynamicMap.permute(): specified (iycells,ixcells) do not match footprint of current pixelization")
 
-        src = torch.from_numpy(old[iycells, ixcells] // (3 * 64 * 64)).to(_device())
-        arr = self._unstable_arr.reshape(self.ncells_curr, 3 * 64 * 64)[src].reshape(-1).contiguous()
-        lp.cell_offsets_cpu[:, :] = new
-        lp.copy_cpu_offsets_to_gpu()
-        self.pixelization_is_finalized = True
-        self._padded_arr = self._unstable_arr = arr
-        self.ncells_allocated = self.ncells_curr
+        # Gather the cells into the new order on the GPU (one kernel), then rewrite the table.
+        order = torch.from_numpy(np.ascontiguousarray(src // self._CELL_FLOATS)).to(self._store.device)
+        moved = torch.empty((n, 3, 64, 64), dtype=torch.float32, device=self._store.device)
+        if n > 0:
+            _ext.cell_broadcast(moved.view(-1), self._store[:n].reshape(-1), order, stream=_stream())
+        table[iy, ix] = np.arange(n, dtype=np.int64) * self._CELL_FLOATS
+        self._table.copy_cpu_offsets_to_gpu()
+        self._store = moved
+        self._state = "fixed"
 
     def finalize(self):
-        """Ends the TOD loop; returns the LocalMap."""
-        if self.map_is_finalized:
+        """Ends accumulation and returns the LocalMap (the DynamicMap gives up its storage)."""
+        if self._state == "done":
             raise RuntimeError("Double call to DynamicMap.finalize() [not currently allowed]")
-        lp = self._unstable_pixelization
-        if not self.pixelization_is_finalized:
-            lp.copy_gpu_offsets_to_cpu()
-            self.pixelization_is_finalized = True
-        arr = self._unstable_arr
-        if self.ncells_allocated > self.ncells_curr:
-            arr = arr.clone()
-        assert lp.npix == self.ncells_curr * 64 * 64
-        ret = LocalMap(lp, arr)
-        self._padded_arr = self._unstable_arr = None
-        self.map_is_finalized = True
-        return ret
+        self._layout()
+        arr = self._store[:self._ncells].reshape(-1)
+        if self._store.shape[0] > self._ncells:
+            arr = arr.clone()          # drop the unused capacity
+        self._store = None
+        self._state = "done"
+        return LocalMap(self._table, arr)
 
     def randomly_permute(self, rng=None):
-        """Test aid: a random cell order."""
+        """Test aid: puts the present cells in a random order (see permute())."""
         rng = np.random.default_rng() if rng is None else rng
-        self._unstable_pixelization.copy_gpu_offsets_to_cpu()
-        iy, ix = np.where(self._unstable_pixelization.cell_offsets_cpu >= 0)
-        perm = rng.permutation(self.ncells_curr)
-        self.permute(iy[perm], ix[perm])
+        iy, ix = np.nonzero(self._layout() >= 0)
+        p = rng.permutation(iy.size)
+        self.permute(iy[p], ix[p])
 
 
 def cell_broadcast(dst, src, index_map):
@@ -707,21 +737,20 @@ BatchConfig = _ext.BatchConfig
 
 def read_xpointing_npzfile(filename):
     """
-    Reads an xpointing array from the reference's Nov-2023 .npz format (key 'xpointing', rows
-    ordered (x, y, alpha)); returns float32 (3, ndet, nt_padded) with rows (y, x, alpha) and nt
-    padded to a multiple of 32 by repeating the last sample. Reference: gpu_mm/gpu_mm.py:1305-1329.
+    Reads an xpointing array stored in the reference's Nov-2023 .npz format: key 'xpointing',
+    shape (3, ndet, nt) with rows ordered (x, y, alpha). Returns a host array of shape
+    (3, ndet, nt_padded), same dtype, with rows in this library's order (y, x, alpha) and nt
+    padded up to a multiple of 32 by repeating each detector's last sample.
+    Reference: gpu_mm/gpu_mm.py:1303-1329.
     """
     with np.load(filename, allow_pickle=False) as f:
         xp = f['xpointing']
-    assert (xp.ndim == 3) and (xp.shape[0] == 3)
-    ndet, ntu = xp.shape[1], xp.shape[2]
-    ntp = 32 * ((ntu + 31) // 32)
-    out = np.zeros((3, ndet, ntp), dtype=xp.dtype)
-    out[0, :, :ntu] = xp[1, :, :]
-    out[1, :, :ntu] = xp[0, :, :]
-    out[2, :, :ntu] = xp[2, :, :]
-    out[:, :, ntu:] = out[:, :, ntu - 1].reshape((3, -1, 1))
-    return out
+    if xp.ndim != 3 or xp.shape[0] != 3 or xp.shape[2] == 0:
+        raise RuntimeError(f"read_xpointing_npzfile({filename!r}): expected 'xpointing' of shape (3, ndet, nt>0),"
+                           f" got {xp.shape}")
+    pad = (-xp.shape[2]) % 32
+    # (x, y, alpha) -> (y, x, alpha); edge padding repeats the last sample of every row.
+    return np.pad(xp[[1, 0, 2]], ((0, 0), (0, 0), (0, pad)), mode="edge")
 
 
 class ToyPointing:

@@ -12,6 +12,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+# ORACLE_LIB selects another build of the same library (the AddressSanitizer one: make asan).
+_LIB_OVERRIDE = os.environ.get("ORACLE_LIB")
 
 _lib = None
 
@@ -33,9 +35,12 @@ def build():
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            build()
-        L = ctypes.CDLL(LIB_PATH)
+        if _LIB_OVERRIDE:
+            L = ctypes.CDLL(os.path.abspath(_LIB_OVERRIDE))
+        else:
+            if not os.path.exists(LIB_PATH):
+                build()
+            L = ctypes.CDLL(LIB_PATH)
         L.orc_locate_f32.argtypes = [ctypes.c_float, ctypes.c_float, _int, _int, _int, _vp, _vp]
         L.orc_locate_f32.restype = _u32
         L.orc_locate_f64.argtypes = [ctypes.c_double, ctypes.c_double, _int, _int, _int, _vp, _vp]
@@ -98,6 +103,16 @@ def _ptr(a):
     return a.ctypes.data
 
 
+def _check_len(name, a, n):
+    """The C loops index caller arrays without bounds checks: every length is checked here."""
+    if a.size != n:
+        raise ValueError(f"pyoracle: {name} has {a.size} elements, expected {n}")
+
+
+def _check_map(lmap, pix):
+    _check_len("lmap", lmap, 3 * pix.npix)
+
+
 def locate(y, x, ny, nx, periodic, f64=False):
     out = np.zeros(4, dtype=np.int32)
     d = np.zeros(2, dtype=np.float64 if f64 else np.float32)
@@ -112,6 +127,8 @@ def map2tod(lmap, xp, pix, partial=False, tod=None, accum=False, f64=False, nthr
     dt = np.float64 if f64 else np.float32
     lmap = np.ascontiguousarray(lmap, dtype=dt).reshape(-1)
     out = np.zeros(n, dtype=dt) if tod is None else np.array(tod, dtype=dt).reshape(-1)
+    _check_map(lmap, pix)
+    _check_len("tod", out, n)
     p = pix.c()
     if nthreads is not None:
         assert not f64
@@ -129,6 +146,8 @@ def tod2map(tod, xp, pix, lmap=None, partial=False, f64=False, nthreads=None):
     dt = np.float64 if f64 else np.float32
     tod = np.ascontiguousarray(tod, dtype=dt).reshape(-1)
     out = np.zeros(3 * pix.npix, dtype=dt) if lmap is None else np.array(lmap, dtype=dt).reshape(-1)
+    _check_len("tod", tod, n)
+    _check_map(out, pix)
     p = pix.c()
     if nthreads is not None:
         assert not f64
@@ -145,9 +164,12 @@ def response_map2tod(lmap, xp, resp, pix, partial=False, tod=None, accum=False):
     xp = np.ascontiguousarray(xp, dtype=np.float32).reshape(3, -1)
     n = xp.shape[1]
     resp = np.ascontiguousarray(resp, dtype=np.float32)
-    assert resp.ndim == 2 and resp.shape[0] == 2 and n % resp.shape[1] == 0
+    if not (resp.ndim == 2 and resp.shape[0] == 2 and resp.shape[1] > 0 and n % resp.shape[1] == 0):
+        raise ValueError(f"pyoracle: response must have shape (2, ndet) with ndet dividing {n}")
     lmap = np.ascontiguousarray(lmap, dtype=np.float64).reshape(-1)
     out = np.zeros(n) if tod is None else np.array(tod, dtype=np.float64).reshape(-1)
+    _check_map(lmap, pix)
+    _check_len("tod", out, n)
     p = pix.c()
     err = lib().orc_response_map2tod_f64(_ptr(out), _ptr(lmap), _ptr(xp), _ptr(resp), resp.shape[1], n,
                                          ctypes.byref(p), int(partial), int(accum))
@@ -159,9 +181,12 @@ def response_tod2map(tod, xp, resp, pix, lmap=None, partial=False):
     xp = np.ascontiguousarray(xp, dtype=np.float32).reshape(3, -1)
     n = xp.shape[1]
     resp = np.ascontiguousarray(resp, dtype=np.float32)
-    assert resp.ndim == 2 and resp.shape[0] == 2 and n % resp.shape[1] == 0
+    if not (resp.ndim == 2 and resp.shape[0] == 2 and resp.shape[1] > 0 and n % resp.shape[1] == 0):
+        raise ValueError(f"pyoracle: response must have shape (2, ndet) with ndet dividing {n}")
     tod = np.ascontiguousarray(tod, dtype=np.float64).reshape(-1)
     out = np.zeros(3 * pix.npix) if lmap is None else np.array(lmap, dtype=np.float64).reshape(-1)
+    _check_len("tod", tod, n)
+    _check_map(out, pix)
     p = pix.c()
     err = lib().orc_response_tod2map_f64(_ptr(out), _ptr(tod), _ptr(xp), _ptr(resp), resp.shape[1], n,
                                          ctypes.byref(p), int(partial))

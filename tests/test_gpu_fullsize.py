@@ -105,3 +105,31 @@ def test_first_detectors_vs_oracle(full, gmm, orc, dev):
     assert err == 0
     got = t[:k].cpu().numpy().astype(np.float64).reshape(-1)
     assert np.sum(np.abs(got - ref)) / np.sum(np.abs(ref)) < 1e-5
+
+
+@pytest.mark.parametrize("d0", [0, 1000, NDET - 32])
+def test_plan_bit_exact_on_detector_ranges(full, orc, d0):
+    """Entries of 32 detectors' lines, taken from the full-size plan (1024 planner blocks of 16384
+    lines, multi-pass radix sort over all 2e7 entries), against the oracle's plan of those 32
+    detectors alone: a line's cells and flags depend on its own samples only, and the canonical
+    (cell, line) order restricted to these lines is their own canonical order."""
+    k = 32
+    lpd = NT // 32                                   # lines per detector
+    mt = full["plan"].get_plan_mt()
+    icl = (mt >> np.uint64(20)) & np.uint64((1 << 26) - 1)
+    lo, hi = np.uint64(d0 * lpd), np.uint64((d0 + k) * lpd)
+    sub = mt[(icl >= lo) & (icl < hi)]
+    xk = full["xp"][:, d0:d0 + k].cpu().numpy()
+    ref, err = orc.plan_build(xk, full["ny"], full["nx"], False)
+    assert err == 0
+    ref = ref + (np.uint64(d0 * lpd) << np.uint64(20))  # the subset's line numbers start at d0's
+    print(f"  detectors {d0}..{d0 + k - 1}: {len(sub)} entries", flush=True)
+    assert np.array_equal(sub, ref)
+    # nmt_cumsum of the planner blocks these detectors fill (2 detectors per block of 16384 lines)
+    cum = full["pre"].get_nmt_cumsum()
+    blk = (d0 * lpd) // 16384
+    assert d0 * lpd % 16384 == 0 and full["pre"].ncl_per_threadblock == 16384
+    per_block = np.bincount(((ref >> np.uint64(20)) & np.uint64((1 << 26) - 1)).astype(np.int64) // 16384,
+                            minlength=blk + k // 2)[blk:blk + k // 2]
+    before = 0 if blk == 0 else int(cum[blk - 1])
+    assert np.array_equal(cum[blk:blk + k // 2].astype(np.int64), before + np.cumsum(per_block))

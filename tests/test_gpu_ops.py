@@ -232,9 +232,16 @@ def test_bad_pointing_raises(gmm, dev):
     for p in (plan, None):
         with pytest.raises(RuntimeError, match="y-value"):
             gmm.map2tod(tg, mg, xg, p)
+    # Plan-free calls have no plan to defer their errors to: they raise in deferred mode too.
+    with gmm.deferred_errflags():
+        with pytest.raises(RuntimeError, match="y-value"):
+            gmm.map2tod(tg, mg, xg, None)
+        with pytest.raises(RuntimeError, match="y-value"):
+            gmm.tod2map(mg, tg, xg, None)
     # The flags were cleared by the failed check: a good call succeeds.
     xg[0, 5] = 10.0
     gmm.map2tod(tg, mg, xg, plan)
+    gmm.map2tod(tg, mg, xg, None)
 
 
 def test_plan_from_other_pointing_raises(gmm, dev):
@@ -682,6 +689,10 @@ def test_partial_last_line(gmm, orc, dev, nt, periodic):
     assert np.array_equal(pre.get_nmt_cumsum(), cum)
     ref_mt, err = orc.plan_build(xp, ny, nx, periodic)
     assert err == 0 and np.array_equal(plan.get_plan_mt(), ref_mt)
+    # The brute-force tester stops at nsamp inside the partial last line (gpu_mm/tests.py:227-254).
+    tester = gmm.PointingPlanTester(pre, xg)
+    assert np.array_equal(tester.nmt_cumsum, cum)
+    assert np.array_equal(tester.sorted_mt, np.sort(ref_mt & np.uint64((1 << 46) - 1)))
 
     lp = gmm.LocalPixelization.make_rectangle(ny, nx, periodic_xcoord=periodic)
     pix = orc.Pix.rectangle(ny, nx, periodic)
@@ -779,3 +790,70 @@ def test_halo_rule_right_edge(gmm, orc, dev, nx, periodic):
     out.arr.zero_()
     b.tod2map(out, debug=True)
     assert l1rel(to_cpu(out.arr), mref32) < 1e-6
+
+
+# ------------------------------------------------------------------------------------------
+# tod2map on TODs far from N(0,1): glitches of 1e4 .. 1e9 times the noise inside one cell, an
+# offset TOD (every deposit of one sign), a TOD that is zero except for a short stretch. The
+# fixed-point quantum follows the bulk of the samples, so the FP64 oracle bar holds pixel by
+# pixel, also next to the glitches.
+
+def _heavy_tail_tods(rng, shape, xp):
+    base = rng.standard_normal(shape).astype(F32)
+    ndet, nt = shape
+    out = {}
+    for amp in (1e4, 1e6, 1e9):
+        d = base.copy()
+        # four glitches of detector 5 within 200 samples (one or two cells), one elsewhere
+        for k, s in enumerate((30000, 30050, 30111, 30190)):
+            d[5, s] = amp * (1 + 0.25 * k) * (-1) ** k
+        d[40, 1234] = -amp
+        out[f"glitch{amp:.0e}"] = d
+    out["offset"] = (3.0 + 0.1 * base).astype(F32)
+    d = np.zeros(shape, F32)
+    d[:, 20000:20800] = base[:, 20000:20800]
+    out["sparse"] = d
+    return out
+
+
+@pytest.mark.parametrize("kind", ["glitch1e+04", "glitch1e+06", "glitch1e+09", "offset", "sparse"])
+def test_tod2map_heavy_tailed_tod(gmm, orc, dev, kind):
+    d = synth.make_cfg0(seed=0)
+    xp, ny, nx = d["xpointing"], d["ny"], d["nx"]
+    print(f"\n[heavy-tailed tod2map] {kind} xpointing {xp.shape}", flush=True)
+    rng = np.random.default_rng(88)
+    tod = _heavy_tail_tods(rng, xp.shape[1:], xp)[kind]
+    xg = to_gpu(xp, dev)
+    plan = gmm.PointingPlan(gmm.PointingPrePlan(xg, ny, nx, periodic_xcoord=False), xg)
+    lp = gmm.LocalPixelization.make_rectangle(ny, nx, periodic_xcoord=False)
+    pix = orc.Pix.rectangle(ny, nx, False)
+    ref, err = orc.tod2map(tod.astype(np.float64), xp, pix, f64=True)
+    assert err == 0
+    # Bulk reference scale: the map of the TOD without its glitches.
+    bulk = tod.copy()
+    bulk[np.abs(bulk) > 100] = 0.0
+    rms = float(np.sqrt(np.mean(orc.tod2map(bulk.astype(np.float64), xp, pix, f64=True)[0] ** 2)))
+    # Pixels holding a glitch's deposit: |value| well above the bulk (below 10 x rms the float
+    # rounding of a value stays under 1e-6 rms).
+    hot = np.abs(ref) > 10 * rms
+
+    def check(got, what):
+        e_bulk = float(np.max(np.abs(got - ref)[~hot])) / rms
+        e_hot = float(np.max((np.abs(got - ref) / np.abs(ref))[hot])) if hot.any() else 0.0
+        print(f"  {what}: max error / map rms (bulk pixels) {e_bulk:.2e}; max relative error (glitch"
+              f" pixels, {int(hot.sum())}) {e_hot:.2e}; l1rel {l1rel(got, ref):.2e}", flush=True)
+        assert e_bulk < 1e-5
+        assert e_hot < 1e-6
+        assert l1rel(got, ref) < 1e-6
+
+    out = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
+    gmm.tod2map(out, to_gpu(tod, dev), xg, plan, debug=True)
+    check(to_cpu(out.arr), "planned")
+
+    # The batched operator shares the accumulation (one TOD, and the same TOD twice).
+    tg = to_gpu(tod, dev)
+    b = gmm.PlanBatch(lp, [plan, plan], [xg, xg], [tg, tg])
+    out2 = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
+    b.tod2map(out2, debug=True)
+    got2 = to_cpu(out2.arr) / 2
+    check(got2, "batched (2 x same TOD) / 2")

@@ -439,3 +439,83 @@ def test_response_reference_check_and_adjoint(orc):
     atd, _ = orc.response_tod2map(d, xp, resp, pix)
     lhs, rhs = np.dot(am, d), np.dot(m, atd)
     assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + np.linalg.norm(am) * np.linalg.norm(d))
+
+
+# ------------------------------------------------------------------------------------------
+# The OpenMP variants (the multi-core CPU baseline) against the single-thread oracle.
+
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("nthreads", [1, 3, 8])
+def test_multithreaded_variants_match_single_thread(orc, periodic, nthreads):
+    rng = np.random.default_rng(40 + nthreads)
+    ny, nx = 200, 300
+    pix = orc.Pix.rectangle(ny, nx, periodic)
+    n = 3 * 32 * 1000 + 17                      # not a multiple of the thread count or of 32
+    xp = random_pointing(rng, n, ny, nx, periodic)
+    m = rng.standard_normal(3 * pix.npix).astype(F32)
+    t0 = rng.standard_normal(n).astype(F32)
+    for accum in (False, True):
+        ref, e1 = orc.map2tod(m, xp, pix, tod=t0, accum=accum)
+        got, e2 = orc.map2tod(m, xp, pix, tod=t0, accum=accum, nthreads=nthreads)
+        assert e1 == e2 == 0
+        assert np.array_equal(got, ref)        # per-sample work: identical bits
+    d = rng.standard_normal(n).astype(F32)
+    m0 = rng.standard_normal(3 * pix.npix).astype(F32)
+    ref, e1 = orc.tod2map(d, xp, pix, lmap=m0)
+    got, e2 = orc.tod2map(d, xp, pix, lmap=m0, nthreads=nthreads)
+    assert e1 == e2 == 0
+    # private maps summed afterwards: a different float summation order
+    assert np.sum(np.abs(got - ref)) / np.sum(np.abs(ref)) < 1e-6
+    # and the FP64 evaluation on the same inputs
+    r64, _ = orc.tod2map(d.astype(np.float64), xp, pix, lmap=m0.astype(np.float64), f64=True)
+    assert np.sum(np.abs(got - r64)) / np.sum(np.abs(r64)) < 1e-6
+
+
+def test_multithreaded_errors_and_partial(orc):
+    rng = np.random.default_rng(44)
+    ny, nx = 128, 128
+    off = -np.ones((2, 2), dtype=np.int64)
+    for i, (cy, cx) in enumerate([(0, 0), (0, 1), (1, 0)]):
+        off[cy, cx] = 3 * 4096 * i
+    pix = orc.Pix(ny, nx, off, 64, 4096, False)
+    xp = random_pointing(rng, 4096, ny, nx, False)
+    xp[0, 100] = ny + 5.0                       # one bad y
+    m = rng.standard_normal(3 * pix.npix).astype(F32)
+    for partial in (False, True):
+        ref, e1 = orc.map2tod(m, xp, pix, partial=partial)
+        got, e2 = orc.map2tod(m, xp, pix, partial=partial, nthreads=4)
+        assert e1 == e2 and (e1 & orc.BAD_Y)
+        assert bool(e1 & orc.NOT_IN_PIX) == (not partial)
+        d = rng.standard_normal(4096).astype(F32)
+        ref, e1 = orc.tod2map(d, xp, pix, partial=partial)
+        got, e2 = orc.tod2map(d, xp, pix, partial=partial, nthreads=4)
+        assert e1 == e2
+
+
+# ------------------------------------------------------------------------------------------
+# The wrappers refuse arrays the C loops would overrun.
+
+def test_wrappers_check_lengths(orc):
+    rng = np.random.default_rng(45)
+    pix = orc.Pix.rectangle(128, 128, False)
+    xp = random_pointing(rng, 256, 128, 128, False)
+    m = np.zeros(3 * pix.npix, dtype=F32)
+    resp = np.ones((2, 8), dtype=F32)
+    with pytest.raises(ValueError, match="lmap"):
+        orc.map2tod(m[:-1], xp, pix)
+    with pytest.raises(ValueError, match="tod"):
+        orc.map2tod(m, xp, pix, tod=np.zeros(255, F32))
+    with pytest.raises(ValueError, match="tod"):
+        orc.tod2map(np.zeros(257, F32), xp, pix)
+    with pytest.raises(ValueError, match="lmap"):
+        orc.tod2map(np.zeros(256, F32), xp, pix, lmap=np.zeros(3 * pix.npix - 64, F32))
+    with pytest.raises(ValueError, match="lmap"):
+        orc.tod2map(np.zeros(256, F32), xp, pix, lmap=m[:-1], nthreads=2)
+    with pytest.raises(ValueError, match="lmap"):
+        orc.response_map2tod(m[:-1], xp, resp, pix)
+    with pytest.raises(ValueError, match="tod"):
+        orc.response_tod2map(np.zeros(255), xp, resp, pix)
+    with pytest.raises(ValueError, match="response"):
+        orc.response_tod2map(np.zeros(256), xp, np.ones((2, 7), F32), pix)
+    with pytest.raises(ValueError, match="beyond"):
+        orc.Pix(128, 128, np.array([[0, 12288], [24576, 40960]]), 64, 4096, False)
