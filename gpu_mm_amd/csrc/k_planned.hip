@@ -24,7 +24,8 @@
 // integer (int32; int64 for the rare samples above the bound, so a glitch does not coarsen the
 // quantum of the cell's other samples), and pixel sums are exact in the int64 tile; the flush
 // converts each pixel once, float(sum)/S, which is more accurate than float32 accumulation. The
-// bound is estimated in the kernel (4x the largest |t| in the first lines of the chunk); a
+// bound is estimated in the kernel (4x a robust per-wave bound on |t| over the first lines of
+// the chunk: a few glitches there do not raise it, wave_probe_bound); a
 // chunk holding a value far beyond it runs again with a bound from its exact maximum, and one
 // holding Inf/NaN runs again with float LDS atomics (t2m_accumulate). The halo
 // row and columns, and the cell's own first row and first HALO_X columns (other blocks' halo
@@ -912,10 +913,10 @@ __device__ inline void t2m_quad_fixed(unsigned long long *__restrict__ acc, cons
 }
 
 // Largest |t| (times the gains) over the first 8 plan entries of [wb, we), for the calling
-// lane (lane 8*li + q reads quad q of entry wb + li).
+// lane (lane 8*li + q reads quad q of entry wb + li); 'act' tells whether the lane read samples.
 template<bool Vec, bool HasResp, typename TT>
 __device__ inline float t2m_probe(const TT *__restrict__ tod, const uint64_t *__restrict__ plan_mt, int64_t nsamp,
-                                  uint32_t wb, uint32_t we, const Gains &gn)
+                                  uint32_t wb, uint32_t we, const Gains &gn, bool &act)
 {
     constexpr bool Q16 = !std::is_same<TT, float>::value;
     using IO = TodIO<TT, Vec>;
@@ -923,12 +924,14 @@ __device__ inline float t2m_probe(const TT *__restrict__ tod, const uint64_t *__
     const int li = lane >> 3, q = lane & 7;
     const uint32_t ncl = uint32_t((nsamp + 31) >> 5);     // the last line may be partial
     float m = 0.0f;
+    act = false;
     const uint32_t idx = wb + uint32_t(li);
     if (idx < we) {
         const uint32_t icl = uint32_t(plan_mt[idx] >> ICL_SHIFT) & ICL_MASK;
         const int64_t s0 = int64_t(icl) * LINE + 4 * q;
         // Groups past the end of a partial last line hold no sample (nothing to read).
         if (icl < ncl && s0 < nsamp) {
+            act = true;
             const uint32_t nv = uint32_t(min<int64_t>(4, nsamp - s0));
             const float4 t4 = IO::load_n(tod + s0, nv);
             float tr[4], pr[4], sc[4];
@@ -944,6 +947,31 @@ __device__ inline float t2m_probe(const TT *__restrict__ tod, const uint64_t *__
         }
     }
     return m;
+}
+
+// Robust bound of a wave's probe: over the lanes that read samples (act), the smallest power of
+// two that exceeds the largest |t| of at least 3/4 of them, as float bits (0 when no lane read
+// anything, +Inf when that bound is not a finite float). A few glitches among the probed samples
+// (up to a quarter of the lanes) leave the bound at the level of the bulk, so the cell keeps a
+// fine fixed-point quantum; the glitches themselves take the 64-bit path of t2m_entries().
+// Binary search over the 8-bit exponent field with wave ballots (8 steps, wave-uniform).
+__device__ inline uint32_t wave_probe_bound(float m, bool act)
+{
+    const int nact = __popcll(__ballot(act));
+    if (nact == 0)
+        return 0u;
+    const int need = (3 * nact + 3) / 4;
+    const int ex = int(__float_as_uint(m) >> 23);          // m >= 0: the biased exponent
+    int lo = 0, hi = 255;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (__popcll(__ballot(act && ex <= mid)) >= need)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    // Every float with biased exponent lo is below 2^(lo - 126), whose bits are (lo + 1) << 23.
+    return (lo >= 254) ? FINITE_LIMIT : uint32_t(lo + 1) << 23;
 }
 
 // tod2map of plan entries [wb, we) of one TOD by the calling wave into the block's halo tile:
@@ -1205,7 +1233,7 @@ __device__ inline void t2m_flush(float *__restrict__ lmap, const unsigned long l
     }
 }
 
-// Block-shared words of t2m_accumulate(): the probe maximum (float bits), and per round a redo
+// Block-shared words of t2m_accumulate(): the probe bound (float bits), and per round a redo
 // word (largest offending |t|, as float bits) and a count of outlier groups.
 enum { W_MAX = 0, W_REDO = 1, W_WIDE = 4, T2M_WORDS = 7 };
 
@@ -1223,14 +1251,15 @@ __device__ inline uint32_t wave_sum(uint32_t v)
 
 // Deposits of one block's work unit (nentries plan entries) into its halo tile, in fixed point.
 // Before the call the tile is clear (fixed-point form), words[W_MAX] holds the float bits of the
-// block's probe maximum, the other words are zero, and the block has synchronized.
+// block's probe bound (the largest wave_probe_bound() of its waves), the other words are zero,
+// and the block has synchronized.
 // run(finite, scale, limit, mrun, tsum, nwide, chk) deposits the calling wave's samples
 // (t2m_entries()). With a bound 2^(e+1) on the bulk of |t|, deposits are w * t * S with
 // S = 2^(27 - e): a group below the bound converts each (3x3-window) pixel sum to int32 (|.| <
 // 2^30); a group above it (an outlier such as a glitch) deposits each contribution as int64,
 // exactly, without coarsening the quantum of the other samples of the cell. The tile sums in
 // int64; every conversion rounds to nearest. The bound starts from an estimate, 4 x the probe
-// maximum. A value beyond the 64-bit range (2^WIDE_BITS times the bound) or a non-finite one
+// bound. A value beyond the 64-bit range (2^WIDE_BITS times the bound) or a non-finite one
 // makes the block discard its tile and run again: with float atomics when the exact maximum M
 // is not finite; else with the bound at M when at least one group in 8 was an outlier (the
 // estimate was far too low, e.g. zeros at the probed lines), or at 2^-WIDE_BITS M otherwise
@@ -1322,8 +1351,9 @@ planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, con
     fill_sincos_table(sct, T2M_THREADS);
     __syncthreads();
     {
-        const float m = t2m_probe<Vec, HasResp, TT>(tod, plan_mt, nsamp, wb, we, gn);
-        const uint32_t mw = wave_max(__float_as_uint(m));
+        bool act;
+        const float m = t2m_probe<Vec, HasResp, TT>(tod, plan_mt, nsamp, wb, we, gn, act);
+        const uint32_t mw = wave_probe_bound(m, act);
         if (lane == 0)
             atomicMax(&s_words[W_MAX], mw);
     }
@@ -1505,16 +1535,16 @@ batched_tod2map_kernel(float *__restrict__ lmap, const BatchTod *__restrict__ to
         // Probe: the first lines of the wave's first segment.
         {
             float m = 0.0f;
-            bool first = true;
+            bool first = true, act = false;
             batch_wave_segments(segs, it, wb, we, [&](const BatchSeg &sg, uint32_t b, uint32_t e) {
                 if (first) {
                     const BatchTod td = tods[sg.tod];
                     m = t2m_probe<true, HasResp, TT>(static_cast<const TT *>(td.tod), td.plan_mt, td.nsamp, b, e,
-                                                   batch_gains(td));
+                                                   batch_gains(td), act);
                     first = false;
                 }
             });
-            const uint32_t mw = wave_max(__float_as_uint(m));
+            const uint32_t mw = wave_probe_bound(m, act);
             if (lane == 0)
                 atomicMax(&s_words[W_MAX], mw);
         }
