@@ -777,6 +777,34 @@ int gmm_expand_dynamic_map(uint32_t *global_ncells_gpu, int64_t *cell_offsets_gp
     });
 }
 
+int gmm_expand_dynamic_map_noalloc(uint32_t *global_ncells_gpu, int64_t *cell_offsets_gpu, int64_t nycells,
+                                   int64_t nxcells, const uint64_t *plan_mt_gpu, int64_t nmt, uint32_t *new_ncells,
+                                   void *stream_)
+{
+    return guarded([&] {
+        const char *where = "expand_dynamic_map_noalloc";
+        require_ptr(global_ncells_gpu, where, "global_ncells");
+        require_ptr(cell_offsets_gpu, where, "cell_offsets");
+        if (nycells <= 0 || nycells > 1024 || nxcells <= 0 || nxcells > 1024)
+            fail(std::string(where) + ": expected 0 < nycells, nxcells <= 1024");
+        if (nycells * nxcells > expand_dynamic_noalloc_max_cells())
+            fail(std::string(where) + ": the cell grid exceeds the scratch array's capacity of "
+                 + std::to_string(expand_dynamic_noalloc_max_cells()) + " cells");
+        if (nmt < 0 || nmt > int64_t(UINT32_MAX) - 1)
+            fail(std::string(where) + ": bad nmt");
+        if (nmt > 0)
+            require_ptr(plan_mt_gpu, where, "plan_mt");
+        hipStream_t stream = as_stream(stream_);
+        launch_expand_dynamic_noalloc(global_ncells_gpu, cell_offsets_gpu, nycells, nxcells, plan_mt_gpu, nmt, stream);
+        if (new_ncells) {
+            uint32_t n = 0;
+            GMM_HIP(hipMemcpyAsync(&n, global_ncells_gpu, 4, hipMemcpyDeviceToHost, stream));
+            GMM_HIP(hipStreamSynchronize(stream));
+            *new_ncells = n;
+        }
+    });
+}
+
 int gmm_lpix_expand_dynamic(gmm_lpix *lpix, const gmm_plan *plan, uint32_t *global_ncells_gpu, uint32_t *new_ncells,
                             void *stream_)
 {

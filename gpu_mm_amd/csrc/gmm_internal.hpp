@@ -225,7 +225,13 @@ void launch_batched_tod2map(const BatchLaunch &bl, float *lmap, const gmm_lpix &
 // cell-key order); scratch of expand_dynamic_scratch_nbytes(nmt) bytes.
 int64_t expand_dynamic_scratch_nbytes(int64_t nmt);
 void launch_expand_dynamic(uint32_t *global_ncells, int64_t *cell_offsets, int64_t nycells, int64_t nxcells,
-                           const uint64_t *plan_mt, int64_t nmt, void *scratch, hipStream_t stream);
+                           const uint64_t *plan_mt, int64_t nmt, void *work, hipStream_t stream);
+// The same result without allocating: per-cell flags over the cell grid in a static device
+// array of the library (at most expand_dynamic_noalloc_max_cells() cells); calls that overlap
+// on one GPU race on that array.
+int64_t expand_dynamic_noalloc_max_cells();
+void launch_expand_dynamic_noalloc(uint32_t *global_ncells, int64_t *cell_offsets, int64_t nycells, int64_t nxcells,
+                                   const uint64_t *plan_mt, int64_t nmt, hipStream_t stream);
 
 // Cell utilities (k_cells.hip). 'bad' (device word) is OR-ed with 1 on an out-of-range index.
 void launch_cell_broadcast(float *dst, int64_t ncells, const float *src, int64_t nsrc, const int64_t *index_map,

@@ -186,6 +186,19 @@ int gmm_response_tod2map(float *local_map_gpu, int64_t local_map_size,
 int gmm_expand_dynamic_map(uint32_t *global_ncells_gpu, int64_t *cell_offsets_gpu, int64_t nycells, int64_t nxcells,
                            const uint64_t *plan_mt_gpu, int64_t nmt, uint32_t *new_ncells, void *stream);
 
+/* No counterpart in the reference: gmm_expand_dynamic_map without any allocation, for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments and the same result (cell indices in
+ * cell-key order). Its scratch is a static device array of the library (about 4 MiB per GPU,
+ * held until the process ends) covering the largest cell grid, 1024 x 1024 cells; a larger grid
+ * is rejected. Calls that overlap on one GPU (several streams or threads) race on that array and
+ * must be serialized by the caller. With new_ncells NULL the call only enqueues work on the
+ * stream (no copy, no synchronization); otherwise it copies the new count to *new_ncells and
+ * synchronizes the stream. */
+int gmm_expand_dynamic_map_noalloc(uint32_t *global_ncells_gpu, int64_t *cell_offsets_gpu, int64_t nycells,
+                                   int64_t nxcells, const uint64_t *plan_mt_gpu, int64_t nmt,
+                                   uint32_t *new_ncells, void *stream);
+
 /* expand_dynamic_map2, include/gpu_mm.hpp:278-283; src_lib/expand_dynamic_map.cu:144-160: the
  * same over a plan's entries, into the LocalPixelization's GPU cell offsets (the full
  * (nycells, nxcells) table). The caller updates npix and the CPU copy, as in the reference. */
