@@ -111,6 +111,65 @@ def run_form(torch, gmm, bufs, lmap, out_map, xp, plan, steps, warmup):
     return t_total, m_ms / m_n * 1e-3, t_ms / t_n * 1e-3
 
 
+def time_sync_errflags(torch, gmm, lmap, out_map, xp, plan, tod, out, steps, warmup):
+    """The drop-in default: every operator call checks its error flags synchronously (one
+    device-to-host copy and stream synchronization per call, as the reference does after every
+    launch, src_lib/map2tod.cu:265-271). Host wall clock over exactly 'steps' steps."""
+    gmm.set_errflag_check(True)
+    try:
+        def step():
+            gmm.map2tod(out, lmap, xp, plan)
+            gmm.tod2map(out_map, tod, xp, plan)
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+    finally:
+        gmm.set_errflag_check(False)
+
+
+def hbm_ceiling(torch, dev, nbytes=1 << 31, reps=10):
+    """Measured bandwidth ceilings on this GPU: a device copy (reads and writes nbytes) and a
+    read-only reduction (torch.sum over nbytes), median over reps, HIP events."""
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    out = {}
+    for name, fn, moved in (("copy", lambda: dst.copy_(src), 2 * nbytes),
+                            ("read", lambda: torch.sum(src), nbytes)):
+        fn()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            ts.append(e0.elapsed_time(e1) * 1e-3)
+        out[name + "_gb_s"] = round(moved / float(np.median(ts)) / 1e9, 1)
+    del src, dst
+    torch.cuda.empty_cache()
+    return out
+
+
+def host_cpu():
+    """Model name and logical CPU count of the host (from /proc/cpuinfo)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(cpu_model=model, host_logical_cpus=os.cpu_count(),
+                usable_cpus=len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None)
+
+
 def form_errors(torch, gmm, w, forms, ndet_err):
     """Map and TOD errors of each form against the FP64 oracle, on the first ndet_err detectors."""
     from oracle import pyoracle as orc
@@ -204,7 +263,8 @@ def cpu_baseline(w, ndet_sample):
     return dict(value=round(st, 5), unit="Gsamp/s", cores=1, kind="port",
                 sample=f"first {ndet_sample} of 2048 detectors x {nt} samples ({n} samples), map2tod + tod2map, "
                        f"single thread (oracle/oracle.c)",
-                multicore=dict(value=round(mt, 5), cores=nthr, threads_kind="OpenMP, tod2map with private maps"))
+                multicore=dict(value=round(mt, 5), cores=nthr, threads_kind="OpenMP, tod2map with private maps"),
+                **host_cpu())
 
 
 def batched_leg(torch, gmm, synth, dev, steps, warmup, ntod, err_ndet):
@@ -521,6 +581,18 @@ def main():
         res[form] = dict(t_total=t_total, ops=ops, gsamp_s=nsamp / (t_total / args.steps) / 1e9)
         log(f"[bench] {form} step: {1e3 * t_total / args.steps:.3f} ms  {res[form]['gsamp_s']:.1f} Gsamp/s")
 
+    # The same FP32 step with the synchronous error-flag check after every call (drop-in default).
+    o32 = torch.empty_like(w["tod"])
+    t_sync = time_sync_errflags(torch, gmm, lmap, out_map, xp, plan, w["tod"], o32, args.steps, args.warmup)
+    del o32
+    errflag_timing = {"deferred_ms_per_step": round(1e3 * res["f32"]["t_total"] / args.steps, 4),
+                      "sync_ms_per_step": round(1e3 * t_sync / args.steps, 4),
+                      "sync_gsamp_s": round(nsamp / (t_sync / args.steps) / 1e9, 3)}
+    log(f"[bench] f32 step with synchronous error-flag checks: {errflag_timing['sync_ms_per_step']:.3f} ms "
+        f"({errflag_timing['sync_gsamp_s']:.1f} Gsamp/s); deferred: {errflag_timing['deferred_ms_per_step']:.3f} ms")
+    ceiling = hbm_ceiling(torch, dev)
+    log(f"[bench] measured ceilings: device copy {ceiling['copy_gb_s']:.0f} GB/s, read-only sum {ceiling['read_gb_s']:.0f} GB/s")
+
     errs = form_errors(torch, gmm, w, forms, min(args.err_ndet, args.ndet)) if args.err_ndet > 0 else {}
     ok = [f for f in forms if f == "f32" or (f in errs and errs[f]["map_rms_rel"] <= MAP_ERR_BAR)]
     chosen = max(ok, key=lambda f: res[f]["gsamp_s"])
@@ -582,6 +654,8 @@ def main():
                               for k, o in v["ops"].items()},
                       "errors": errs.get(f)} for f, v in res.items()},
         "plan_ms": {"preplan": round(t_pre, 3), "plan": round(t_plan, 3)},
+        "errflag_check": errflag_timing,
+        "hbm_measured_ceiling": ceiling,
         "roofline": {
             "kernel": f"planned_{dom}_kernel ({chosen} TOD)",
             "bound": "hbm",
