@@ -326,7 +326,7 @@ int gmm_batch_create(gmm_batch **out, const gmm_lpix *lpix, int32_t ntod, const 
             t.plan_mt = pl->plan_mt;
             t.qmask = pl->qmask;
             t.nsamp = nsamp[i];
-            t.resp_ndet = t.resp_nper = 1;
+            t.resp_ndet = 1;
             if (q16) {
                 require_ptr(tod_scale_gpu[i], ws.c_str(), "tod_scale");
                 const int64_t nr = tod_nrows[i];
@@ -334,7 +334,7 @@ int gmm_batch_create(gmm_batch **out, const gmm_lpix *lpix, int32_t ntod, const 
                     fail(ws + ": tod_scale has " + std::to_string(nr) + " rows, which does not divide nsamp="
                          + std::to_string(nsamp[i]));
                 t.scale = tod_scale_gpu[i];
-                t.scale_nper = uint32_t(nsamp[i] / nr);
+                t.scale_div = make_rowdiv(uint32_t(nsamp[i] / nr));
             }
             b->tods.push_back(t);
 
@@ -389,13 +389,14 @@ int gmm_batch_set_response(gmm_batch *b, const float *const *response_gpu, const
                 check_response(response_gpu[i], resp_ndet[i], tods[i].nsamp, ws.c_str());
                 tods[i].resp = response_gpu[i];
                 tods[i].resp_ndet = uint32_t(resp_ndet[i]);
-                tods[i].resp_nper = uint32_t(tods[i].nsamp / resp_ndet[i]);
+                tods[i].resp_div = make_rowdiv(uint32_t(tods[i].nsamp / resp_ndet[i]));
             }
         }
         else {
             for (BatchTod &t : tods) {
                 t.resp = nullptr;
-                t.resp_ndet = t.resp_nper = 1;
+                t.resp_ndet = 1;
+                t.resp_div = RowDiv();
             }
         }
         GMM_HIP(hipMemcpy(b->tods_dev, tods.data(), tods.size() * sizeof(BatchTod), hipMemcpyHostToDevice));

@@ -218,23 +218,30 @@ struct TodIO
 };
 
 // Per-sample gains. response[2][ndet] (src_lib/map2tod_response.cu, tod2map_response.cu): row 0
-// multiplies T, row 1 multiplies Q and U; sample s belongs to detector s / resp_nper. A 16-bit
-// TOD holds value / scale[s / scale_nper].
+// multiplies T, row 1 multiplies Q and U; sample s belongs to detector s / resp_div.nper. A 16-bit
+// TOD holds value / scale[s / scale_div.nper].
 struct Gains
 {
     const float *__restrict__ resp;
     const float *__restrict__ scale;
-    uint32_t resp_ndet, resp_nper, scale_nper;
+    uint32_t resp_ndet;
+    RowDiv resp_div, scale_div;
 };
 
-// Row of each of the 4 samples s0..s0+3 for rows of 'nper' samples (one division per quad when
-// a quad can cross at most one row boundary). Only the first nv samples exist (nv < 4 for the
-// last group of a TOD whose length is not a multiple of 4): the others take the row of the last.
-__device__ inline void quad_rows(uint32_t s0, uint32_t nper, uint32_t (&r)[4], uint32_t nv = 4)
+// Row of each of the 4 samples s0..s0+3 (s0 a multiple of 4) for rows of d.nper samples. When
+// nper is a multiple of 4 the quad lies in one row (one multiplication); otherwise it crosses at
+// most one row boundary when nper >= 4. Only the first nv samples exist (nv < 4 for the last
+// group of a TOD whose length is not a multiple of 4): the others take the row of the last.
+__device__ inline void quad_rows(uint32_t s0, const RowDiv &d, uint32_t (&r)[4], uint32_t nv = 4)
 {
-    if (nper >= 4) {
-        const uint32_t r0 = s0 / nper;
-        const uint32_t b = (r0 + 1) * nper - s0;    // samples of the quad in row r0
+    const uint32_t r0 = rowdiv(s0, d);
+    if ((d.nper & 3u) == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            r[k] = r0;
+    }
+    else if (d.nper >= 4) {
+        const uint32_t b = (r0 + 1) * d.nper - s0;  // samples of the quad in row r0
 #pragma unroll
         for (int k = 0; k < 4; k++)
             r[k] = r0 + ((min(uint32_t(k), nv - 1) >= b) ? 1u : 0u);
@@ -242,7 +249,7 @@ __device__ inline void quad_rows(uint32_t s0, uint32_t nper, uint32_t (&r)[4], u
     else {
 #pragma unroll
         for (int k = 0; k < 4; k++)
-            r[k] = (s0 + min(uint32_t(k), nv - 1)) / nper;
+            r[k] = rowdiv(s0 + min(uint32_t(k), nv - 1), d);
     }
 }
 
@@ -252,11 +259,21 @@ __device__ inline void quad_resp(const Gains &g, uint32_t s0, float (&tr)[4], fl
 {
     if (HasResp) {
         uint32_t d[4];
-        quad_rows(s0, g.resp_nper, d, nv);
+        quad_rows(s0, g.resp_div, d, nv);
+        if ((g.resp_div.nper & 3u) == 0) {
+            const float t = g.resp[d[0]], p = g.resp[g.resp_ndet + d[0]];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            tr[k] = g.resp[d[k]];
-            pr[k] = g.resp[g.resp_ndet + d[k]];
+            for (int k = 0; k < 4; k++) {
+                tr[k] = t;
+                pr[k] = p;
+            }
+        }
+        else {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                tr[k] = g.resp[d[k]];
+                pr[k] = g.resp[g.resp_ndet + d[k]];
+            }
         }
     }
     else {
@@ -277,10 +294,18 @@ __device__ inline void quad_scale(const Gains &g, uint32_t s0, float (&sc)[4])
     }
     else {
         uint32_t r[4];
-        quad_rows(s0, g.scale_nper, r);
+        quad_rows(s0, g.scale_div, r);
+        if ((g.scale_div.nper & 3u) == 0) {
+            const float v = g.scale[r[0]];
 #pragma unroll
-        for (int k = 0; k < 4; k++)
-            sc[k] = g.scale[r[k]];
+            for (int k = 0; k < 4; k++)
+                sc[k] = v;
+        }
+        else {
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                sc[k] = g.scale[r[k]];
+        }
     }
 }
 
@@ -1452,7 +1477,7 @@ __device__ inline void batch_wave_segments(const BatchSeg *__restrict__ segs, co
 
 __device__ inline Gains batch_gains(const BatchTod &td)
 {
-    return Gains{ td.resp, td.scale, td.resp_ndet, td.resp_nper, td.scale_nper };
+    return Gains{ td.resp, td.scale, td.resp_ndet, td.resp_div, td.scale_div };
 }
 
 template<bool Periodic, typename TT, int NT, bool HasResp = false>
@@ -1613,13 +1638,13 @@ void launch_t2m(float *lmap, const TT *tod, const float *xp, const OpArgs &a, co
 
 Gains make_gains(const OpArgs &a, const TodGains &g)
 {
-    Gains gn{g.response, g.scale, 1u, 1u, 1u};
+    Gains gn{g.response, g.scale, 1u, RowDiv(), RowDiv()};
     if (g.response) {
         gn.resp_ndet = uint32_t(g.resp_ndet);
-        gn.resp_nper = uint32_t(a.nsamp / g.resp_ndet);
+        gn.resp_div = make_rowdiv(uint32_t(a.nsamp / g.resp_ndet));
     }
     if (g.scale)
-        gn.scale_nper = uint32_t(a.nsamp / g.scale_nrows);
+        gn.scale_div = make_rowdiv(uint32_t(a.nsamp / g.scale_nrows));
     return gn;
 }
 
