@@ -31,6 +31,9 @@ HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 CELL_BYTES = 3 * 64 * 64 * 4
 MAP_ERR_BAR = 1e-4
 COPY_REF_FLOATS = (1 << 28) + (3 << 20)
+# 16-bit input TODs of tod2map are encoded with second-order error feedback (encode_tod16): the
+# map error against the FP64 oracle is what the 1e-4 bar applies to.
+TOD16_ERROR_FEEDBACK = 2
 # Algorithmic bytes per sample: 12 B pointing + the TOD element.
 BYTES_PER_SAMPLE = {"f32": 16, "f16": 14, "bf16": 14}
 
@@ -76,7 +79,7 @@ def form_buffers(torch, gmm, form, w, lmap, plan):
     if form == "f32":
         return dict(tod=w["tod"], out=torch.empty_like(w["tod"]), sc_in=None, sc_out=None)
     dt = {"f16": torch.float16, "bf16": torch.bfloat16}[form]
-    t16, sc = gmm.encode_tod16(w["tod"], dt)
+    t16, sc = gmm.encode_tod16(w["tod"], dt, error_feedback=TOD16_ERROR_FEEDBACK)
     # Output scales fitted to the map2tod output (the FP32 result, encoded once).
     o32 = torch.empty_like(w["tod"])
     gmm.map2tod(o32, lmap, w["xp"], plan)
@@ -354,7 +357,7 @@ def batched_leg(torch, gmm, synth, dev, steps, warmup, ntod, err_ndet):
     torch.cuda.synchronize()
     cfg16 = None
     for form, dt in (("f16", torch.float16), ("bf16", torch.bfloat16)):
-        enc_in = [gmm.encode_tod16(t, dt) for t in ins]
+        enc_in = [gmm.encode_tod16(t, dt, error_feedback=TOD16_ERROR_FEEDBACK) for t in ins]
         enc_out = [gmm.encode_tod16(o, dt) for o in outs]
         b16m = gmm.PlanBatch(lp, plans, xps, [e[0] for e in enc_out], scales=[e[1] for e in enc_out])
         b16t = gmm.PlanBatch(lp, plans, xps, [e[0] for e in enc_in], scales=[e[1] for e in enc_in])
@@ -425,7 +428,7 @@ def batched_errors(torch, gmm, lp, lmap, xps, ins, ny, nx, ndet_err):
             dsc = None
         else:
             dt = torch.float16 if form == "f16" else torch.bfloat16
-            enc = [gmm.encode_tod16(t, dt) for t in ts]
+            enc = [gmm.encode_tod16(t, dt, error_feedback=TOD16_ERROR_FEEDBACK) for t in ts]
             src, ssc = [e[0] for e in enc], [e[1] for e in enc]
             o32 = [torch.empty_like(t) for t in ts]
             gmm.PlanBatch(lp, plans, xs, o32).map2tod(lmap)
@@ -628,6 +631,15 @@ def main():
         with open(tfile) as f:
             traffic = json.load(f).get("bytes_per_launch")
     a = r["ops"][dom]["bytes"] / r["ops"][dom]["seconds"]
+    # The plain FP32 operators' roofline, reported beside the chosen form's when they differ.
+    r32 = res["f32"]
+    dom32 = max(r32["ops"], key=lambda k: r32["ops"][k]["seconds"])
+    a32 = r32["ops"][dom32]["bytes"] / r32["ops"][dom32]["seconds"]
+    t32 = None
+    tfile32 = os.path.join(REPO, "profiles", f"traffic_planned_{dom32}_f32_{args.ndet}x{args.nt}.json")
+    if os.path.exists(tfile32):
+        with open(tfile32) as f:
+            t32 = json.load(f).get("bytes_per_launch")
     line = {
         "metric": "Gsamp/s map2tod+tod2map & %HBM peak, chosen vs plain FP32 at <=1e-4 rel err",
         "value": round(r["gsamp_s"], 3),
@@ -645,6 +657,7 @@ def main():
             "workload": "planned map2tod+tod2map, single TOD 2048x262144 on sparse 3x4096x8192 map",
             "ndet": args.ndet, "nt": args.nt, "nsamp": nsamp, "cells": ncells, "plan_nmt": int(plan.plan_nmt),
             "chunks": int(plan.nchunks), "chosen_form": chosen, "tod_storage": chosen,
+            "tod16_error_feedback": TOD16_ERROR_FEEDBACK,
             "map_error_bar": MAP_ERR_BAR, "error_sample": f"first {args.err_ndet} detectors vs FP64 oracle",
         },
         "forms": {f: {"gsamp_s": round(v["gsamp_s"], 2), "ms_per_step": round(1e3 * v["t_total"] / args.steps, 4),
@@ -665,6 +678,16 @@ def main():
             "frac": round(a / HBM_PEAK, 4),
             "traffic": traffic,
             "algorithmic_bytes_per_launch": int(r["ops"][dom]["bytes"]),
+        },
+        "roofline_plain_f32": {
+            "kernel": f"planned_{dom32}_kernel (f32 TOD)",
+            "bound": "hbm",
+            "achieved": round(a32 / 1e9, 1),
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": round(a32 / HBM_PEAK, 4),
+            "traffic": t32,
+            "algorithmic_bytes_per_launch": int(r32["ops"][dom32]["bytes"]),
         },
         "cpu_baseline": cpu,
         "batched": batched,

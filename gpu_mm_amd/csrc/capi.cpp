@@ -875,7 +875,7 @@ int gmm_planned_tod2map_q16(float *local_map_gpu, int64_t local_map_size,
 }
 
 int gmm_tod16_encode(void *out_gpu, int tod_type, float *scale_gpu, const float *tod_gpu, int64_t nrows,
-                     int64_t row_len, void *stream_)
+                     int64_t row_len, int error_feedback, void *stream_)
 {
     return guarded([&] {
         const char *where = "tod16_encode";
@@ -886,10 +886,12 @@ int gmm_tod16_encode(void *out_gpu, int tod_type, float *scale_gpu, const float 
             fail(std::string(where) + ": tod_type must be GMM_TOD_F16 or GMM_TOD_BF16");
         if (nrows < 0 || row_len < 0 || nrows > int64_t(INT32_MAX))
             fail(std::string(where) + ": bad shape");
+        if (error_feedback < 0 || error_feedback > 2)
+            fail(std::string(where) + ": error_feedback must be 0, 1 or 2");
         if (nrows == 0 || row_len == 0)
             return;
         launch_tod16_encode(out_gpu, tod_type == GMM_TOD_F16 ? TOD_F16 : TOD_BF16, scale_gpu, tod_gpu, nrows,
-                            row_len, as_stream(stream_));
+                            row_len, error_feedback, as_stream(stream_));
     });
 }
 

@@ -242,9 +242,9 @@ void launch_local_to_global(float *dst, const float *src, const gmm_lpix &lp, hi
 
 // Per-row 16-bit encoding of a float32 TOD [nrows][row_len]: scale[r] = max(max_s |tod[r][s]| /
 // (0.5 * largest finite value of the type), FLT_MIN), out[r][s] = tod[r][s] / scale[r] rounded
-// to nearest even.
+// to nearest even, with error feedback of order 0 (none), 1 or 2 along each row.
 void launch_tod16_encode(void *out, int tod_type, float *scale, const float *tod, int64_t nrows,
-                         int64_t row_len, hipStream_t stream);
+                         int64_t row_len, int error_feedback, hipStream_t stream);
 // Words of tod2map scratch kept in the plan buffer.
 
 int64_t unplanned_nblocks(int64_t nsamp, int64_t max_blocks);

@@ -518,7 +518,7 @@ struct Graph
     ~Graph() { gmm_graph_destroy(h); }
 };
 
-void tod16_encode(py::object out, py::object scale, py::object tod, uintptr_t stream)
+void tod16_encode(py::object out, py::object scale, py::object tod, int error_feedback, uintptr_t stream)
 {
     const char *where = "tod16_encode";
     Tod16 o = tod16_args(out, scale, where);
@@ -530,7 +530,7 @@ void tod16_encode(py::object out, py::object scale, py::object tod, uintptr_t st
         raise(std::string(where) + ": expected scale to have one entry per TOD row");
     py::gil_scoped_release nogil;
     check(gmm_tod16_encode(o.t.ptr, o.type, static_cast<float *>(o.sc.ptr), static_cast<const float *>(t.ptr), nrows,
-                           nrows ? t.size / nrows : 0, as_stream(stream)));
+                           nrows ? t.size / nrows : 0, error_feedback, as_stream(stream)));
 }
 
 Arr errflags_arr(py::handle o, const char *where)
@@ -882,7 +882,8 @@ PYBIND11_MODULE(_gpu_mm_amd_py, m)
             check(gmm_graph_launch(g.h, as_stream(stream)));
         }, py::arg("stream") = 0);
 
-    m.def("tod16_encode", &tod16_encode, py::arg("out"), py::arg("scale"), py::arg("tod"), py::arg("stream") = 0);
+    m.def("tod16_encode", &tod16_encode, py::arg("out"), py::arg("scale"), py::arg("tod"), py::arg("error_feedback") = 0,
+          py::arg("stream") = 0);
     m.def("unplanned_map2tod", &unplanned_map2tod, py::arg("tod"), py::arg("local_map"), py::arg("xpointing"),
           py::arg("local_pixelization"), py::arg("errflags"), py::arg("partial_pixelization"), py::arg("accum"),
           py::arg("check_errors") = true, py::arg("stream") = 0);

@@ -264,9 +264,12 @@ int gmm_planned_tod2map_q16(float *local_map_gpu, int64_t local_map_size,
 
 /* Encodes a float32 TOD [nrows][row_len] (device) into out_gpu (16-bit, same shape) and
  * scale_gpu (float[nrows]): scale = max(max|row| / (0.5 * largest finite value), FLT_MIN),
- * out = tod / scale rounded to nearest even. */
+ * out = tod / scale rounded to nearest even. error_feedback 1 or 2 carries each sample's rounding
+ * error (first or second order) into the next sample of its row before rounding, which shapes
+ * the encoding error towards high frequencies, where tod2map averages it out; 0 rounds each
+ * sample independently. No counterpart in the reference (float32 TODs only). */
 int gmm_tod16_encode(void *out_gpu, int tod_type, float *scale_gpu, const float *tod_gpu, int64_t nrows,
-                     int64_t row_len, void *stream);
+                     int64_t row_len, int error_feedback, void *stream);
 
 /* ---- Batched operators -------------------------------------------------------------------
  * No counterpart in the reference, which launches one TOD at a time (map2tod.cu:198-272,
