@@ -41,6 +41,7 @@ LIB_SOURCES = [
     ("k_tod16.hip", []),
     ("k_dynmap.hip", []),
     ("k_cells.hip", []),
+    ("k_debug.hip", []),
 ]
 
 LIB_NAME = "libgpu_mm_amd.so"

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <mutex>
 #include <cstring>
+#include <memory>
 #include <sstream>
 
 namespace gmm {
@@ -529,7 +530,6 @@ int gmm_plan_create(gmm_plan **out, const gmm_preplan *pp, const float *xpointin
         require_ptr(xpointing_gpu, where, "xpointing_gpu");
         require_ptr(buf_gpu, where, "buf");
         require_ptr(tmp_buf_gpu, where, "tmp_buf");
-        (void)debug;
         if (nsamp != pp->nsamp) {
             std::ostringstream ss;
             ss << where << ": xpointing has " << nsamp << " samples, preplan was built for " << pp->nsamp;
@@ -585,6 +585,8 @@ int gmm_plan_create(gmm_plan **out, const gmm_preplan *pp, const float *xpointin
             GMM_REQUIRE(plan->nchunks <= pp->max_nchunks,
                         "PointingPlan constructor: work list larger than its bound (internal error)");
             order_chunks(*pp, plan->chunks, plan->nchunks, a, b, scratch, scratch_nbytes, stream);
+            if (debug)
+                check_plan_invariants(*plan, where, stream);
         }
         catch (...) {
             delete plan;
@@ -633,6 +635,14 @@ int gmm_plan_check_errflags(const gmm_plan *plan, const char *where)
     });
 }
 
+int gmm_plan_check_invariants(const gmm_plan *plan, void *stream_)
+{
+    return guarded([&] {
+        require_ptr(plan, "PointingPlan check_invariants", "plan");
+        check_plan_invariants(*plan, "PointingPlan check_invariants", as_stream(stream_));
+    });
+}
+
 // ------------------------------------------------------------------------------------------
 
 int gmm_planned_map2tod(float *tod_gpu, int64_t tod_nsamp,
@@ -652,9 +662,13 @@ int gmm_planned_map2tod(float *tod_gpu, int64_t tod_nsamp,
         check_op_common(lpix, plan, tod_nsamp, xpointing_nsamp, local_map_size, where);
         require_ptr(lpix->offsets_gpu, where, "cell_offsets_gpu");
         OpArgs a{lpix, tod_nsamp, partial_pixelization != 0};
+        std::unique_ptr<OpDebug> dbg(debug ? new OpDebug(plan->pp.plan_nmt, tod_nsamp, stream) : nullptr);
+        a.debug = dbg ? &dbg->bufs : nullptr;
         launch_planned_map2tod(tod_gpu, local_map_gpu, xpointing_gpu, a, *plan, accum != 0, TodGains{}, stream);
         if (debug || check_errors)
             check_plan_errors(*plan, where, stream);
+        if (dbg)
+            dbg->check(where);
     });
 }
 
@@ -674,9 +688,13 @@ int gmm_planned_tod2map(float *local_map_gpu, int64_t local_map_size,
         check_op_common(lpix, plan, tod_nsamp, xpointing_nsamp, local_map_size, where);
         require_ptr(lpix->offsets_gpu, where, "cell_offsets_gpu");
         OpArgs a{lpix, tod_nsamp, partial_pixelization != 0};
+        std::unique_ptr<OpDebug> dbg(debug ? new OpDebug(plan->pp.plan_nmt, tod_nsamp, stream) : nullptr);
+        a.debug = dbg ? &dbg->bufs : nullptr;
         launch_planned_tod2map(local_map_gpu, tod_gpu, xpointing_gpu, a, *plan, TodGains{}, stream);
         if (debug || check_errors)
             check_plan_errors(*plan, where, stream);
+        if (dbg)
+            dbg->check(where);
     });
 }
 
@@ -700,12 +718,16 @@ int gmm_response_map2tod(float *tod_gpu, int64_t tod_nsamp,
         check_response(response_gpu, ndet, tod_nsamp, where);
         require_ptr(lpix->offsets_gpu, where, "cell_offsets_gpu");
         OpArgs a{lpix, tod_nsamp, partial_pixelization != 0};
+        std::unique_ptr<OpDebug> dbg(debug ? new OpDebug(plan->pp.plan_nmt, tod_nsamp, stream) : nullptr);
+        a.debug = dbg ? &dbg->bufs : nullptr;
         TodGains g;
         g.response = response_gpu;
         g.resp_ndet = ndet;
         launch_planned_map2tod(tod_gpu, local_map_gpu, xpointing_gpu, a, *plan, accum != 0, g, stream);
         if (debug || check_errors)
             check_plan_errors(*plan, where, stream);
+        if (dbg)
+            dbg->check(where);
     });
 }
 
@@ -727,12 +749,16 @@ int gmm_response_tod2map(float *local_map_gpu, int64_t local_map_size,
         check_response(response_gpu, ndet, tod_nsamp, where);
         require_ptr(lpix->offsets_gpu, where, "cell_offsets_gpu");
         OpArgs a{lpix, tod_nsamp, partial_pixelization != 0};
+        std::unique_ptr<OpDebug> dbg(debug ? new OpDebug(plan->pp.plan_nmt, tod_nsamp, stream) : nullptr);
+        a.debug = dbg ? &dbg->bufs : nullptr;
         TodGains g;
         g.response = response_gpu;
         g.resp_ndet = ndet;
         launch_planned_tod2map(local_map_gpu, tod_gpu, xpointing_gpu, a, *plan, g, stream);
         if (debug || check_errors)
             check_plan_errors(*plan, where, stream);
+        if (dbg)
+            dbg->check(where);
     });
 }
 

@@ -172,11 +172,43 @@ int64_t build_chunks_generic_scratch(int64_t nmt);
 // ------------------------------------------------------------------------------------------
 // Operator launchers (k_planned.hip, k_unplanned.hip).
 
+// Device counters of a debug launch of the planned operators (k_debug.hip): visits per plan
+// entry, evaluations per sample, halo-tile accesses outside the tile.
+struct DebugBuffers
+{
+    uint32_t *entry_visits = nullptr;   // [nmt]
+    uint32_t *sample_evals = nullptr;   // [nsamp]
+    uint32_t *tile_bad = nullptr;       // one word
+};
+
 struct OpArgs
 {
     const gmm_lpix *lpix;
     int64_t nsamp;
     bool partial;
+    const DebugBuffers *debug = nullptr;   // debug=True on a float32 TOD: instrumented kernels
+};
+
+// Debug checks (k_debug.hip). check_plan_invariants() raises if the plan breaks its order, flag
+// or work-list invariants. OpDebug allocates and clears the operator counters; check() raises
+// if an entry was not walked exactly once, a sample not evaluated exactly once, or a tile access
+// fell outside the tile. Both synchronize the stream.
+void check_plan_invariants(const gmm_plan &plan, const char *where, hipStream_t stream);
+
+class OpDebug
+{
+public:
+    OpDebug(int64_t nmt, int64_t nsamp, hipStream_t stream);
+    ~OpDebug();
+    OpDebug(const OpDebug &) = delete;
+    OpDebug &operator=(const OpDebug &) = delete;
+    void check(const char *where);
+    DebugBuffers bufs;
+
+private:
+    int64_t nmt_, nsamp_;
+    hipStream_t stream_;
+    void *mem_ = nullptr;
 };
 
 // Kernel timer (measurement aid): event pairs around the main planned kernels while enabled.

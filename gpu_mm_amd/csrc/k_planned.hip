@@ -47,6 +47,25 @@ constexpr int HROWS = 65;           // halo tile rows: the cell plus row 0 of th
 constexpr int HS = 64 + HALO_X;     // halo tile row stride: the cell plus HALO_X columns to the right
 constexpr int HPLANE = HROWS * HS;  // pixels per halo tile plane
 
+// Device-side invariant counters of a debug launch of the planned operators (debug=True), checked
+// by the host after the launch (k_debug.hip): visits of each plan entry (exactly one wave walks
+// each entry), evaluations of each sample (exactly one block evaluates each sample: the ownership
+// and halo rules), and halo-tile accesses outside the tile (none). Unused outside debug launches.
+struct Dbg
+{
+    uint32_t *entry_visits;   // [nmt]
+    uint32_t *sample_evals;   // [nsamp]
+    uint32_t *tile_bad;       // one word
+};
+
+// Tile offset o of a sample's (iy0, ix0) corner: its four corners o, o+1, o+HS, o+HS+1 lie in
+// the tile.
+__device__ inline void dbg_tile(const Dbg &d, int o, int extra = HS + 1)
+{
+    if (!(o >= 0 && o + extra < HPLANE))
+        atomicAdd(d.tile_bad, 1u);
+}
+
 constexpr int M2T_THREADS = 512;
 constexpr int T2M_THREADS = 1024;
 
@@ -578,10 +597,10 @@ __device__ inline bool quad_small_angles(const float4 &a4)
     return !__any(!(m <= 1.0e5f));
 }
 
-template<bool Periodic, bool HasResp, bool Small>
+template<bool Periodic, bool HasResp, bool Small, bool Debug = false>
 __device__ inline float m2t_sample(const float4 *__restrict__ tile, const float2 *__restrict__ tab, float y, float x,
                                    float alpha, const Geom &g, int ybase, int xbase, unsigned lxlim, float tr, float pr,
-                                   Checks &chk, bool &owned)
+                                   Checks &chk, bool &owned, const Dbg &dbg = Dbg{})
 {
     int iy0, ix0;
     float dy, dx;
@@ -591,18 +610,22 @@ __device__ inline float m2t_sample(const float4 *__restrict__ tile, const float2
     float sn, cs;
     sincos2<Small>(tab, alpha, sn, cs);
     const int o = owned ? (ly * HS + lx) : 0;
+    if constexpr (Debug) {
+        if (owned)
+            dbg_tile(dbg, o);
+    }
     return m2t_interp<HasResp>(tile, o, dy, dx, cs, sn, tr, pr);
 }
 
 // map2tod of plan entries [wb, we) of one TOD by the calling wave, with the halo tile of the
 // block's cell (whose top-left pixel is (ybase, xbase)) in LDS. Every sample is written once, by
 // the block that owns it: no atomics and no zeroing.
-template<bool Periodic, bool Vec, bool HasResp, typename TT>
+template<bool Periodic, bool Vec, bool HasResp, typename TT, bool Debug = false>
 __device__ inline void m2t_entries(TT *__restrict__ tod, const float *__restrict__ xp,
                                    const uint64_t *__restrict__ plan_mt, const uint16_t *__restrict__ qmask,
                                    int64_t nsamp, uint32_t wb, uint32_t we, const float4 *__restrict__ tile,
                                    const float2 *__restrict__ sct, const Geom &g, int ybase, int xbase, bool accum,
-                                   const Gains &gn, Checks &chk, uint32_t &err)
+                                   const Gains &gn, Checks &chk, uint32_t &err, const Dbg &dbg = Dbg{})
 {
     const int lane = threadIdx.x & 63;
     const int li = lane >> 3, q = lane & 7;
@@ -624,6 +647,10 @@ __device__ inline void m2t_entries(TT *__restrict__ tod, const float *__restrict
             const bool act = (idx < cnt) && (icl < ncl);
             if (idx < cnt && icl >= ncl)
                 err |= ERR_INCONSISTENT_NMT;
+            if constexpr (Debug) {
+                if (q == 0 && idx < cnt)
+                    atomicAdd(dbg.entry_visits + (base + idx), 1u);
+            }
             // Quads without a sample this entry evaluates are neither loaded nor evaluated.
             if (!act || !((qm >> q) & 1u))
                 continue;
@@ -641,24 +668,30 @@ __device__ inline void m2t_entries(TT *__restrict__ tod, const float *__restrict
             if (quad_small_angles(a4)) {
                 // Every angle of the wave's groups takes the table path: straight-line code, so
                 // the LDS reads of the four samples overlap.
-                v.x = m2t_sample<Periodic, HasResp, true>(tile, sct, y4.x, x4.x, a4.x, g, ybase, xbase, lxlim, tr[0], pr[0], chk, o0);
-                v.y = m2t_sample<Periodic, HasResp, true>(tile, sct, y4.y, x4.y, a4.y, g, ybase, xbase, lxlim, tr[1], pr[1], chk, o1);
-                v.z = m2t_sample<Periodic, HasResp, true>(tile, sct, y4.z, x4.z, a4.z, g, ybase, xbase, lxlim, tr[2], pr[2], chk, o2);
-                v.w = m2t_sample<Periodic, HasResp, true>(tile, sct, y4.w, x4.w, a4.w, g, ybase, xbase, lxlim, tr[3], pr[3], chk, o3);
+                v.x = m2t_sample<Periodic, HasResp, true, Debug>(tile, sct, y4.x, x4.x, a4.x, g, ybase, xbase, lxlim, tr[0], pr[0], chk, o0, dbg);
+                v.y = m2t_sample<Periodic, HasResp, true, Debug>(tile, sct, y4.y, x4.y, a4.y, g, ybase, xbase, lxlim, tr[1], pr[1], chk, o1, dbg);
+                v.z = m2t_sample<Periodic, HasResp, true, Debug>(tile, sct, y4.z, x4.z, a4.z, g, ybase, xbase, lxlim, tr[2], pr[2], chk, o2, dbg);
+                v.w = m2t_sample<Periodic, HasResp, true, Debug>(tile, sct, y4.w, x4.w, a4.w, g, ybase, xbase, lxlim, tr[3], pr[3], chk, o3, dbg);
             }
             else {
-                v.x = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.x, x4.x, a4.x, g, ybase, xbase, lxlim, tr[0], pr[0], chk, o0);
+                v.x = m2t_sample<Periodic, HasResp, false, Debug>(tile, sct, y4.x, x4.x, a4.x, g, ybase, xbase, lxlim, tr[0], pr[0], chk, o0, dbg);
                 __builtin_amdgcn_sched_barrier(0);
-                v.y = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.y, x4.y, a4.y, g, ybase, xbase, lxlim, tr[1], pr[1], chk, o1);
+                v.y = m2t_sample<Periodic, HasResp, false, Debug>(tile, sct, y4.y, x4.y, a4.y, g, ybase, xbase, lxlim, tr[1], pr[1], chk, o1, dbg);
                 __builtin_amdgcn_sched_barrier(0);
-                v.z = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.z, x4.z, a4.z, g, ybase, xbase, lxlim, tr[2], pr[2], chk, o2);
+                v.z = m2t_sample<Periodic, HasResp, false, Debug>(tile, sct, y4.z, x4.z, a4.z, g, ybase, xbase, lxlim, tr[2], pr[2], chk, o2, dbg);
                 __builtin_amdgcn_sched_barrier(0);
-                v.w = m2t_sample<Periodic, HasResp, false>(tile, sct, y4.w, x4.w, a4.w, g, ybase, xbase, lxlim, tr[3], pr[3], chk, o3);
+                v.w = m2t_sample<Periodic, HasResp, false, Debug>(tile, sct, y4.w, x4.w, a4.w, g, ybase, xbase, lxlim, tr[3], pr[3], chk, o3, dbg);
             }
             // Samples past the end of the TOD (read as zeros) are not written.
             o1 = o1 && nv > 1;
             o2 = o2 && nv > 2;
             o3 = o3 && nv > 3;
+            if constexpr (Debug) {
+                if (o0) atomicAdd(dbg.sample_evals + s0, 1u);
+                if (o1) atomicAdd(dbg.sample_evals + s0 + 1, 1u);
+                if (o2) atomicAdd(dbg.sample_evals + s0 + 2, 1u);
+                if (o3) atomicAdd(dbg.sample_evals + s0 + 3, 1u);
+            }
             if constexpr (!std::is_same<TT, float>::value) {
                 // The hardware reciprocal (1 ulp) is exact enough ahead of rounding to 16 bits.
                 v.x *= __builtin_amdgcn_rcpf(sc[0]);
@@ -685,13 +718,13 @@ __device__ inline void m2t_entries(TT *__restrict__ tod, const float *__restrict
     }
 }
 
-template<bool Periodic, bool Vec, bool HasResp, typename TT>
+template<bool Periodic, bool Vec, bool HasResp, typename TT, bool Debug = false>
 __global__ void __launch_bounds__(M2T_THREADS, 4)
 planned_map2tod_kernel(TT *__restrict__ tod, const float *__restrict__ lmap, const float *__restrict__ xp,
                        const int64_t *__restrict__ cell_offsets, const uint64_t *__restrict__ plan_mt,
                        const Chunk *__restrict__ chunks, const uint16_t *__restrict__ qmask,
                        uint32_t *__restrict__ err_word, int64_t nsamp, int ny, int nx, int nycells, int nxcells,
-                       int64_t ystride, int64_t polstride, bool partial, bool accum, Gains gn)
+                       int64_t ystride, int64_t polstride, bool partial, bool accum, Gains gn, Dbg dbg)
 {
     __shared__ float4 tile[HPLANE];                       // 77 KiB
     __shared__ float2 sct[SCT];
@@ -711,8 +744,8 @@ planned_map2tod_kernel(TT *__restrict__ tod, const float *__restrict__ lmap, con
 
     uint32_t wb, we;
     wave_slice(ch, wave, M2T_THREADS / WAVE, wb, we);
-    m2t_entries<Periodic, Vec, HasResp, TT>(tod, xp, plan_mt, qmask, nsamp, wb, we, tile, sct, g, ybase, xbase, accum,
-                                            gn, chk, err);
+    m2t_entries<Periodic, Vec, HasResp, TT, Debug>(tod, xp, plan_mt, qmask, nsamp, wb, we, tile, sct, g, ybase, xbase,
+                                                   accum, gn, chk, err, dbg);
     report_err(err | chk.bits(g), err_word);
 }
 
@@ -777,22 +810,27 @@ __device__ inline void deposit4(unsigned long long *__restrict__ acc, float *__r
 // One sample of tod2map: locate; if the block evaluates it (its (iy0, ix0) corner lies in the
 // block's cell, or at local column < lxlim of the cell to the right), deposit all four corners
 // into the halo tile.
-template<bool Periodic, bool HasResp, int Dep, bool Small>
-__device__ inline void t2m_sample(unsigned long long *__restrict__ acc, float *__restrict__ facc,
+template<bool Periodic, bool HasResp, int Dep, bool Small, bool Debug = false>
+__device__ inline bool t2m_sample(unsigned long long *__restrict__ acc, float *__restrict__ facc,
                                   const float2 *__restrict__ tab, float y, float x, float alpha, float t, const Geom &g,
-                                  int ybase, int xbase, unsigned lxlim, float tr, float pr, Checks &chk)
+                                  int ybase, int xbase, unsigned lxlim, float tr, float pr, Checks &chk,
+                                  const Dbg &dbg = Dbg{})
 {
     int iy0, ix0;
     float dy, dx;
     locate_op<Periodic>(y, x, g, chk, iy0, ix0, dy, dx);
     const int ly = iy0 - ybase, lx = ix0 - xbase;
-    if ((unsigned(ly) < 64u) && (unsigned(lx) < lxlim)) {
+    const bool owned = (unsigned(ly) < 64u) && (unsigned(lx) < lxlim);
+    if (owned) {
         float sn, cs;
         sincos2<Small>(tab, alpha, sn, cs);
         const float ts = HasResp ? t * tr : t;
         const float ps = HasResp ? t * pr : t;
+        if constexpr (Debug)
+            dbg_tile(dbg, ly * HS + lx);
         deposit4<Dep>(acc, facc, ly * HS + lx, dy, dx, ts, ps * cs, ps * sn);
     }
+    return owned;
 }
 
 // sin and cos of x0 + d from s0 = sin x0, c0 = cos x0 for |d| <= 0.0185, where cos d = 1 - d^2/2
@@ -812,11 +850,12 @@ __device__ inline void angle_step(float s0, float c0, float d, float &sn, float 
 // added with one LDS atomic per touched pixel and plane (18 for a group that stays in one pixel
 // row, at most 27) instead of 12 per sample. A pixel's sum of at most four contributions, each
 // below 2^28 in magnitude, rounds to an int32. Other groups deposit sample by sample.
-template<bool Periodic, bool HasResp, bool Small>
+template<bool Periodic, bool HasResp, bool Small, bool Debug = false>
 __device__ inline void t2m_quad_fixed(unsigned long long *__restrict__ acc, const float2 *__restrict__ tab,
                                       const float4 &y4, const float4 &x4, const float4 &a4, const float4 &t4,
                                       const Geom &g, int ybase, int xbase, unsigned lxlim, const float (&tr)[4],
-                                      const float (&pr)[4], Checks &chk)
+                                      const float (&pr)[4], Checks &chk, const Dbg &dbg = Dbg{}, int64_t samp0 = 0,
+                                      uint32_t nv = 4, bool count = false)
 {
     const float yv[4] = { y4.x, y4.y, y4.z, y4.w }, xv[4] = { x4.x, x4.y, x4.z, x4.w };
     const float av[4] = { a4.x, a4.y, a4.z, a4.w }, tv[4] = { t4.x, t4.y, t4.z, t4.w };
@@ -836,6 +875,11 @@ __device__ inline void t2m_quad_fixed(unsigned long long *__restrict__ acc, cons
             ey = max(ey, ly[k]);
             bx = min(bx, lx[k]);
             ex = max(ex, lx[k]);
+        }
+        if constexpr (Debug) {
+            // Samples past the end of the TOD read as zeros: they deposit nothing.
+            if (count && own[k] && uint32_t(k) < nv)
+                atomicAdd(dbg.sample_evals + samp0 + k, 1u);
         }
     }
     if (ey < 0)
@@ -891,6 +935,8 @@ __device__ inline void t2m_quad_fixed(unsigned long long *__restrict__ acc, cons
         // the tile: ey + 1 <= 64 and ex + 1 <= lxlim <= HS - 1.
         const bool row2 = ey > by, col2 = ex > bx;
         const int o = by * HS + bx;
+        if constexpr (Debug)
+            dbg_tile(dbg, o, (row2 ? 2 : 1) * HS + (col2 ? 2 : 1));   // the window's touched extent
 #pragma unroll
         for (int p = 0; p < 3; p++)
 #pragma unroll
@@ -932,6 +978,8 @@ __device__ inline void t2m_quad_fixed(unsigned long long *__restrict__ acc, cons
             }
             const float ts = HasResp ? tv[k] * tr[k] : tv[k];
             const float ps = HasResp ? tv[k] * pr[k] : tv[k];
+            if constexpr (Debug)
+                dbg_tile(dbg, ly[k] * HS + lx[k]);
             deposit4<DEP_FIXED>(acc, nullptr, ly[k] * HS + lx[k], dy[k], dx[k], ts, ps * cs, ps * sn);
         }
     }
@@ -1005,13 +1053,14 @@ __device__ inline uint32_t wave_probe_bound(float m, bool act)
 // outlier) deposits sample by sample in 64-bit fixed point and counts in nwide. Tracks per lane
 // the largest |t| of its outlier groups (mrun) and a sum that is NaN iff some deposited t is NaN
 // (tsum).
-template<bool Periodic, bool Vec, bool HasResp, typename TT>
+template<bool Periodic, bool Vec, bool HasResp, typename TT, bool Debug = false>
 __device__ inline void t2m_entries(unsigned long long *__restrict__ acc, float *__restrict__ facc,
                                    const TT *__restrict__ tod, const float *__restrict__ xp,
                                    const uint64_t *__restrict__ plan_mt, const uint16_t *__restrict__ qmask,
                                    int64_t nsamp, uint32_t wb, uint32_t we, const float2 *__restrict__ sct,
                                    const Geom &g, int ybase, int xbase, const Gains &gn, bool finite, float scale,
-                                   float limit, float &mrun, float &tsum, uint32_t &nwide, Checks &chk, uint32_t &err)
+                                   float limit, float &mrun, float &tsum, uint32_t &nwide, Checks &chk, uint32_t &err,
+                                   const Dbg &dbg = Dbg{}, bool count = false)
 {
     constexpr bool Q16 = !std::is_same<TT, float>::value;
     using IO = TodIO<TT, Vec>;
@@ -1034,6 +1083,10 @@ __device__ inline void t2m_entries(unsigned long long *__restrict__ acc, float *
             const bool act = (idx < cnt) && (icl < ncl);
             if (idx < cnt && icl >= ncl)
                 err |= ERR_INCONSISTENT_NMT;
+            if constexpr (Debug) {
+                if (count && q == 0 && idx < cnt)
+                    atomicAdd(dbg.entry_visits + (base + idx), 1u);
+            }
             if (!act || !((qm >> q) & 1u))
                 continue;
             const int64_t s0 = int64_t(icl) * LINE + 4 * q;
@@ -1076,25 +1129,37 @@ __device__ inline void t2m_entries(unsigned long long *__restrict__ acc, float *
                 t4.w *= scale;
                 if (tmax < limit) {
                     if (quad_small_angles(a4))
-                        t2m_quad_fixed<Periodic, HasResp, true>(acc, sct, y4, x4, a4, t4, g, ybase, xbase, lxlim, tr, pr, chk);
+                        t2m_quad_fixed<Periodic, HasResp, true, Debug>(acc, sct, y4, x4, a4, t4, g, ybase, xbase, lxlim, tr, pr, chk, dbg, s0, nv, count);
                     else
-                        t2m_quad_fixed<Periodic, HasResp, false>(acc, sct, y4, x4, a4, t4, g, ybase, xbase, lxlim, tr, pr, chk);
+                        t2m_quad_fixed<Periodic, HasResp, false, Debug>(acc, sct, y4, x4, a4, t4, g, ybase, xbase, lxlim, tr, pr, chk, dbg, s0, nv, count);
                 }
                 else {
                     // Only an outlier group can exceed the 64-bit range: mrun is tracked here.
                     mrun = fmaxf(mrun, tmax);
                     nwide++;
-                    t2m_sample<Periodic, HasResp, DEP_WIDE, false>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, lxlim, tr[0], pr[0], chk);
-                    t2m_sample<Periodic, HasResp, DEP_WIDE, false>(acc, facc, sct, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, lxlim, tr[1], pr[1], chk);
-                    t2m_sample<Periodic, HasResp, DEP_WIDE, false>(acc, facc, sct, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, lxlim, tr[2], pr[2], chk);
-                    t2m_sample<Periodic, HasResp, DEP_WIDE, false>(acc, facc, sct, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, lxlim, tr[3], pr[3], chk);
+                    bool ob[4];
+                    ob[0] = t2m_sample<Periodic, HasResp, DEP_WIDE, false, Debug>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, lxlim, tr[0], pr[0], chk, dbg);
+                    ob[1] = t2m_sample<Periodic, HasResp, DEP_WIDE, false, Debug>(acc, facc, sct, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, lxlim, tr[1], pr[1], chk, dbg);
+                    ob[2] = t2m_sample<Periodic, HasResp, DEP_WIDE, false, Debug>(acc, facc, sct, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, lxlim, tr[2], pr[2], chk, dbg);
+                    ob[3] = t2m_sample<Periodic, HasResp, DEP_WIDE, false, Debug>(acc, facc, sct, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, lxlim, tr[3], pr[3], chk, dbg);
+                    if constexpr (Debug) {
+                        for (int k = 0; k < 4; k++)
+                            if (count && ob[k] && uint32_t(k) < nv)
+                                atomicAdd(dbg.sample_evals + s0 + k, 1u);
+                    }
                 }
             }
             else {
-                t2m_sample<Periodic, HasResp, DEP_FLOAT, false>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, lxlim, tr[0], pr[0], chk);
-                t2m_sample<Periodic, HasResp, DEP_FLOAT, false>(acc, facc, sct, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, lxlim, tr[1], pr[1], chk);
-                t2m_sample<Periodic, HasResp, DEP_FLOAT, false>(acc, facc, sct, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, lxlim, tr[2], pr[2], chk);
-                t2m_sample<Periodic, HasResp, DEP_FLOAT, false>(acc, facc, sct, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, lxlim, tr[3], pr[3], chk);
+                bool ob[4];
+                ob[0] = t2m_sample<Periodic, HasResp, DEP_FLOAT, false, Debug>(acc, facc, sct, y4.x, x4.x, a4.x, t4.x, g, ybase, xbase, lxlim, tr[0], pr[0], chk, dbg);
+                ob[1] = t2m_sample<Periodic, HasResp, DEP_FLOAT, false, Debug>(acc, facc, sct, y4.y, x4.y, a4.y, t4.y, g, ybase, xbase, lxlim, tr[1], pr[1], chk, dbg);
+                ob[2] = t2m_sample<Periodic, HasResp, DEP_FLOAT, false, Debug>(acc, facc, sct, y4.z, x4.z, a4.z, t4.z, g, ybase, xbase, lxlim, tr[2], pr[2], chk, dbg);
+                ob[3] = t2m_sample<Periodic, HasResp, DEP_FLOAT, false, Debug>(acc, facc, sct, y4.w, x4.w, a4.w, t4.w, g, ybase, xbase, lxlim, tr[3], pr[3], chk, dbg);
+                if constexpr (Debug) {
+                    for (int k = 0; k < 4; k++)
+                        if (count && ob[k] && uint32_t(k) < nv)
+                            atomicAdd(dbg.sample_evals + s0 + k, 1u);
+                }
             }
         }
     }
@@ -1278,7 +1343,7 @@ __device__ inline uint32_t wave_sum(uint32_t v)
 // Before the call the tile is clear (fixed-point form), words[W_MAX] holds the float bits of the
 // block's probe bound (the largest wave_probe_bound() of its waves), the other words are zero,
 // and the block has synchronized.
-// run(finite, scale, limit, mrun, tsum, nwide, chk) deposits the calling wave's samples
+// run(finite, scale, limit, mrun, tsum, nwide, chk, first) deposits the calling wave's samples
 // (t2m_entries()). With a bound 2^(e+1) on the bulk of |t|, deposits are w * t * S with
 // S = 2^(27 - e): a group below the bound converts each (3x3-window) pixel sum to int32 (|.| <
 // 2^30); a group above it (an outlier such as a glitch) deposits each contribution as int64,
@@ -1309,7 +1374,7 @@ __device__ inline bool t2m_accumulate(unsigned long long *__restrict__ acc, uint
         float tsum = 0.0f;                               // NaN iff some deposited t is NaN
         uint32_t nwide = 0;                              // outlier groups of this lane
         Checks chk;
-        run(finite, scale, limit, mrun, tsum, nwide, chk);
+        run(finite, scale, limit, mrun, tsum, nwide, chk, pass == 0);
         err |= chk.bits(g);
         if (!finite)
             break;
@@ -1342,13 +1407,13 @@ __device__ inline bool t2m_accumulate(unsigned long long *__restrict__ acc, uint
     return finite;
 }
 
-template<bool Periodic, bool Vec, bool HasResp, typename TT>
+template<bool Periodic, bool Vec, bool HasResp, typename TT, bool Debug = false>
 __global__ void __launch_bounds__(T2M_THREADS)
 planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, const float *__restrict__ xp,
                        const int64_t *__restrict__ cell_offsets, const uint64_t *__restrict__ plan_mt,
                        const Chunk *__restrict__ chunks, const uint16_t *__restrict__ qmask,
                        uint32_t *__restrict__ err_word, int64_t nsamp, int ny, int nx, int nycells, int nxcells,
-                       int64_t ystride, int64_t polstride, bool partial, Gains gn)
+                       int64_t ystride, int64_t polstride, bool partial, Gains gn, Dbg dbg)
 {
     __shared__ __align__(16) unsigned long long acc[3 * HPLANE];       // 116 KiB
     __shared__ float2 sct[SCT];
@@ -1389,9 +1454,10 @@ planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, con
     float inv_scale;
     const bool finite = t2m_accumulate<T2M_THREADS>(
         acc, s_words, ch.end - ch.begin, g,
-        [&](bool fin, float scale, float limit, float &mrun, float &tsum, uint32_t &nwide, Checks &chk) {
-            t2m_entries<Periodic, Vec, HasResp, TT>(acc, facc, tod, xp, plan_mt, qmask, nsamp, wb, we, sct, g, ybase,
-                                                    xbase, gn, fin, scale, limit, mrun, tsum, nwide, chk, err);
+        [&](bool fin, float scale, float limit, float &mrun, float &tsum, uint32_t &nwide, Checks &chk, bool first) {
+            t2m_entries<Periodic, Vec, HasResp, TT, Debug>(acc, facc, tod, xp, plan_mt, qmask, nsamp, wb, we, sct, g,
+                                                           ybase, xbase, gn, fin, scale, limit, mrun, tsum, nwide, chk,
+                                                           err, dbg, first);
         },
         inv_scale, err, threadIdx.x);
     t2m_flush<T2M_THREADS>(lmap, acc, facc, h, (ch.flags & CHUNK_SHARED) != 0, finite, inv_scale, ystride,
@@ -1580,7 +1646,7 @@ batched_tod2map_kernel(float *__restrict__ lmap, const BatchTod *__restrict__ to
         float inv_scale;
         const bool finite = t2m_accumulate<T2M_THREADS>(
             acc, s_words, it.nent, g,
-            [&](bool fin, float scale, float limit, float &mrun, float &tsum, uint32_t &nwide, Checks &chk) {
+            [&](bool fin, float scale, float limit, float &mrun, float &tsum, uint32_t &nwide, Checks &chk, bool) {
                 batch_wave_segments(segs, it, wb, we, [&](const BatchSeg &sg, uint32_t b, uint32_t e) {
                     const BatchTod td = tods[sg.tod];
                     t2m_entries<Periodic, true, HasResp, TT>(acc, facc, static_cast<const TT *>(td.tod), td.xp,
@@ -1610,10 +1676,19 @@ void launch_m2t(TT *tod, const float *lmap, const float *xp, const OpArgs &a, co
     const bool vec = (((reinterpret_cast<uintptr_t>(tod) | reinterpret_cast<uintptr_t>(xp)) & 15) == 0) && (a.nsamp % 4 == 0);
     auto k = lp.periodic_xcoord ? (vec ? planned_map2tod_kernel<true, true, HasResp, TT> : planned_map2tod_kernel<true, false, HasResp, TT>)
                                 : (vec ? planned_map2tod_kernel<false, true, HasResp, TT> : planned_map2tod_kernel<false, false, HasResp, TT>);
+    Dbg dbg{ nullptr, nullptr, nullptr };
+    if constexpr (std::is_same<TT, float>::value) {
+        // debug=True: the instrumented instantiation (float32 TODs).
+        if (a.debug) {
+            dbg = Dbg{ a.debug->entry_visits, a.debug->sample_evals, a.debug->tile_bad };
+            k = lp.periodic_xcoord ? (vec ? planned_map2tod_kernel<true, true, HasResp, TT, true> : planned_map2tod_kernel<true, false, HasResp, TT, true>)
+                                   : (vec ? planned_map2tod_kernel<false, true, HasResp, TT, true> : planned_map2tod_kernel<false, false, HasResp, TT, true>);
+        }
+    }
     kernel_timer_mark(TK_MAP2TOD, false, stream);
     k<<<grid, block, 0, stream>>>(tod, lmap, xp, lp.offsets_gpu, plan.plan_mt, plan.chunks, plan.qmask, err_word,
                                   a.nsamp, int(lp.nypix_global), int(lp.nxpix_global), int(lp.nycells),
-                                  int(lp.nxcells), lp.ystride, lp.polstride, a.partial, accum, gn);
+                                  int(lp.nxcells), lp.ystride, lp.polstride, a.partial, accum, gn, dbg);
     GMM_HIP(hipGetLastError());
     kernel_timer_mark(TK_MAP2TOD, true, stream);
 }
@@ -1628,10 +1703,18 @@ void launch_t2m(float *lmap, const TT *tod, const float *xp, const OpArgs &a, co
     const bool vec = (((reinterpret_cast<uintptr_t>(tod) | reinterpret_cast<uintptr_t>(xp)) & 15) == 0) && (a.nsamp % 4 == 0);
     auto k = lp.periodic_xcoord ? (vec ? planned_tod2map_kernel<true, true, HasResp, TT> : planned_tod2map_kernel<true, false, HasResp, TT>)
                                 : (vec ? planned_tod2map_kernel<false, true, HasResp, TT> : planned_tod2map_kernel<false, false, HasResp, TT>);
+    Dbg dbg{ nullptr, nullptr, nullptr };
+    if constexpr (std::is_same<TT, float>::value) {
+        if (a.debug) {
+            dbg = Dbg{ a.debug->entry_visits, a.debug->sample_evals, a.debug->tile_bad };
+            k = lp.periodic_xcoord ? (vec ? planned_tod2map_kernel<true, true, HasResp, TT, true> : planned_tod2map_kernel<true, false, HasResp, TT, true>)
+                                   : (vec ? planned_tod2map_kernel<false, true, HasResp, TT, true> : planned_tod2map_kernel<false, false, HasResp, TT, true>);
+        }
+    }
     kernel_timer_mark(TK_TOD2MAP, false, stream);
     k<<<grid, block, 0, stream>>>(lmap, tod, xp, lp.offsets_gpu, plan.plan_mt, plan.chunks, plan.qmask, err_word,
                                   a.nsamp, int(lp.nypix_global), int(lp.nxpix_global), int(lp.nycells),
-                                  int(lp.nxcells), lp.ystride, lp.polstride, a.partial, gn);
+                                  int(lp.nxcells), lp.ystride, lp.polstride, a.partial, gn, dbg);
     GMM_HIP(hipGetLastError());
     kernel_timer_mark(TK_TOD2MAP, true, stream);
 }

@@ -677,6 +677,10 @@ PYBIND11_MODULE(_gpu_mm_amd_py, m)
         .def("_check_errflags", [](const PointingPlan &p, const std::string &where) {
             check(gmm_plan_check_errflags(p.h, where.c_str()));
         }, py::arg("where"))
+        .def("_check_invariants", [](const PointingPlan &p, uintptr_t stream) {
+            py::gil_scoped_release nogil;
+            check(gmm_plan_check_invariants(p.h, as_stream(stream)));
+        }, py::arg("stream") = 0)
         .def("get_plan_mt", [](const PointingPlan &p) {
             int64_t n = p.info().plan_nmt;
             py::array_t<uint64_t> out(n);

@@ -611,6 +611,9 @@ class PointingPlan(_ext.PointingPlan):
     Attributes: nsamp, nypix_global, nxpix_global, periodic_xcoord, plan_nmt, nchunks,
     nmlines; get_plan_mt() returns the entries (uint64, bit layout of the reference).
 
+    With debug=True the constructor also checks the plan's invariants on the device
+    (check_invariants()), and raises RuntimeError if one is broken.
+
     Reference: gpu_mm/gpu_mm.py:1262-1299; src_lib/PointingPlan.cu:237-312.
     """
 
@@ -620,6 +623,18 @@ class PointingPlan(_ext.PointingPlan):
         if tmp_buf is None:
             tmp_buf = torch.empty(preplan.plan_constructor_tmp_nbytes, dtype=torch.uint8, device=_device())
         _ext.PointingPlan.__init__(self, preplan, xpointing_gpu, buf, tmp_buf, bool(debug), _stream())
+        self.buf = buf
+
+    def check_invariants(self):
+        """
+        Checks the plan on the device and raises RuntimeError, counting each kind of violation,
+        unless: entries are in (cell, line) order with each pair once; every line index is in
+        range and every line has an entry; mflag marks exactly the entries of multi-cell lines
+        and each such line has exactly one zflag; the operators' work list covers every entry
+        exactly once, each chunk within one cell. (The reference asserts its iterator and cell
+        enumerator invariants on the device under Debug, include/plan_iterator.hpp:64-141.)
+        """
+        self._check_invariants(_stream())
 
     def check_errflags(self, where="PointingPlan"):
         """Raises if an operator call since the last check reported a pointing error."""

@@ -112,7 +112,9 @@ int gmm_preplan_get_nmt_cumsum(const gmm_preplan *pp, uint32_t *out_cpu, int64_t
  * bound at gpu_mm_pybind11.cu:133-153). 'buf' (>= plan_nbytes, 128-byte aligned) is kept by
  * the plan and must not be shared; 'tmp_buf' (>= plan_constructor_tmp_nbytes) is used only
  * during construction. Layout of buf: uint64 plan_mt[nmt] at offset 0, then uint32 err[...]
- * at align128(8*nmt), then the operator work list. Blocks on the stream once.
+ * at align128(8*nmt), then the operator work list. Blocks on the stream once. debug=1 also
+ * checks the plan's invariants on the device (gmm_plan_check_invariants) and fails if one is
+ * broken (PointingPlan.cu:272-283 launches its Debug planner kernel instead).
  * --------------------------------------------------------------------------------------- */
 
 typedef struct {
@@ -133,12 +135,23 @@ int gmm_plan_get_plan_mt(const gmm_plan *plan, uint64_t *out, int64_t n, int out
                          void *stream);
 /* PointingPlan.cu:331-335: synchronous check of the plan's error flags. */
 int gmm_plan_check_errflags(const gmm_plan *plan, const char *where);
+/* The plan invariants that a debug=1 gmm_plan_create checks after construction, on demand:
+ * entries in (cell, line) order with each pair once, line indices in range, every line present,
+ * the mflag/zflag rule, and a work list covering every entry once within one cell per chunk.
+ * Returns nonzero with a message counting each kind of violation. The reference asserts its
+ * iterator and enumerator invariants on the device under Debug (plan_iterator.hpp:64-68, 91-103,
+ * 137-141; gpu_mm_internals.hpp:224-235); this is the same role as a check that never traps. */
+int gmm_plan_check_invariants(const gmm_plan *plan, void *stream);
 
 /* ---------------------------------------------------------------------------------------
  * Operators. map2tod: tod (=, or += when accum) A * map. tod2map: map += A^T * tod.
  * 'debug' makes the call check error flags synchronously (as the reference always does);
  * with check_errors == 0 and debug == 0 the call is asynchronous and errors are only
- * visible through gmm_plan_check_errflags().
+ * visible through gmm_plan_check_errflags(). With debug = 1 the planned float32 operators also
+ * run an instrumented build of their kernel and fail unless every plan entry was walked by
+ * exactly one wave, every sample evaluated by exactly one block (the ownership and halo rules)
+ * and every halo-tile access stayed inside the tile (the reference's Debug asserts, map2tod.cu:
+ * 105-109, tod2map.cu:134-139, plan_iterator.hpp:64-141). 16-bit TODs are not instrumented.
  * --------------------------------------------------------------------------------------- */
 
 /* launch_planned_map2tod, include/gpu_mm.hpp:155-165; src_lib/map2tod.cu:194-272. */

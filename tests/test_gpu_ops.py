@@ -920,3 +920,87 @@ def test_tod2map_heavy_tailed_tod(gmm, orc, dev, kind):
     b.tod2map(out2, debug=True)
     got2 = to_cpu(out2.arr) / 2
     check(got2, "batched (2 x same TOD) / 2")
+
+
+# ------------------------------------------------------------------------------------------
+# Debug checks (debug=True): device-side invariants of plans and of the planned operators, the
+# role of the reference's Debug asserts (include/plan_iterator.hpp:64-141,
+# include/gpu_mm_internals.hpp:224-235, src_lib/map2tod.cu:105-109, src_lib/tod2map.cu:134-139).
+
+def test_debug_checks_pass(inst, gmm, dev):
+    """Real plans pass every plan invariant, and the instrumented operators walk every entry
+    once, evaluate every sample once and stay inside the halo tile; their results equal the
+    non-debug kernels' (map2tod bit for bit; tod2map to float rounding: the flush's atomics)."""
+    xg, lp = inst["xg"], inst["lp"]
+    plan = gmm.PointingPlan(inst["pre"], xg, debug=True)
+    plan.check_invariants()
+    rng = np.random.default_rng(71 + inst["seed"])
+    shape = inst["xp"].shape[1:]
+    m = gmm.LocalMap(lp, to_gpu(rng.standard_normal(3 * lp.npix).astype(F32), dev))
+    tod = to_gpu(rng.standard_normal(shape).astype(F32), dev)
+    ndet = shape[0]
+    resp = to_gpu(rng.uniform(0.5, 1.5, (2, ndet)).astype(F32), dev)
+    for response in (None, resp):
+        t_dbg, t_ref = torch.empty_like(tod), torch.empty_like(tod)
+        gmm.map2tod(t_dbg, m, xg, plan, response=response, debug=True)
+        gmm.map2tod(t_ref, m, xg, plan, response=response)
+        assert torch.equal(t_dbg, t_ref)
+        o_dbg = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
+        o_ref = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
+        gmm.tod2map(o_dbg, tod, xg, plan, response=response, debug=True)
+        gmm.tod2map(o_ref, tod, xg, plan, response=response)
+        assert l1rel(to_cpu(o_dbg.arr), to_cpu(o_ref.arr)) < 1e-6
+
+
+def test_debug_checks_partial_last_line(gmm, dev):
+    """A TOD whose length is not a multiple of 32: the samples past the end are neither walked nor
+    counted, every real sample is evaluated exactly once."""
+    d = synth.make_cfg0(seed=0, ndet=4, nt=4096)
+    xp = np.ascontiguousarray(d["xpointing"].reshape(3, -1)[:, :4 * 4096 - 13])
+    xg = to_gpu(xp, dev)
+    plan = gmm.PointingPlan(gmm.PointingPrePlan(xg, d["ny"], d["nx"], periodic_xcoord=False), xg, debug=True)
+    lp = gmm.LocalPixelization.make_rectangle(d["ny"], d["nx"], periodic_xcoord=False)
+    m = gmm.LocalMap(lp, to_gpu(d["map"].reshape(-1), dev))
+    tod = torch.empty(xp.shape[1], dtype=torch.float32, device=dev)
+    gmm.map2tod(tod, m, xg, plan, debug=True)
+    out = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
+    gmm.tod2map(out, tod, xg, plan, debug=True)
+
+
+def _corrupt_entry(plan, dev, i, new_entry):
+    """Overwrites plan entry i (uint64, at byte 8*i of the plan buffer) on the device."""
+    plan.buf[8 * i:8 * i + 8].copy_(torch.from_numpy(np.array([new_entry], dtype=np.uint64).view(np.uint8)).to(dev))
+
+
+def test_debug_checks_detect_corrupted_plans(gmm, dev):
+    """Entries changed on the device after construction (indices stay in range, so every kernel
+    reads and writes inside its buffers): the plan check and the instrumented operators raise."""
+    d = synth.make_cfg0(seed=0, ndet=8, nt=8192)
+    xp, ny, nx = d["xpointing"], d["ny"], d["nx"]
+    xg = to_gpu(xp, dev)
+    pre = gmm.PointingPrePlan(xg, ny, nx, periodic_xcoord=False)
+    lp = gmm.LocalPixelization.make_rectangle(ny, nx, periodic_xcoord=False)
+    m = gmm.LocalMap(lp, to_gpu(d["map"].reshape(-1), dev))
+    tod = torch.empty(xp.shape[1:], dtype=torch.float32, device=dev)
+    mt = gmm.PointingPlan(pre, xg).get_plan_mt()
+    cell = mt & np.uint64((1 << 20) - 1)
+    # An entry that repeats the line of the entry before it in the same cell: out of order, one
+    # line's samples evaluated twice in that cell and another's not at all.
+    i = int(np.nonzero(cell[1:] == cell[:-1])[0][10]) + 1
+    plan = gmm.PointingPlan(pre, xg)
+    _corrupt_entry(plan, dev, i, int(mt[i - 1]))
+    with pytest.raises(RuntimeError, match="plan invariant violated"):
+        plan.check_invariants()
+    with pytest.raises(RuntimeError, match="operator invariant violated"):
+        gmm.map2tod(tod, m, xg, plan, debug=True)
+    out = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
+    with pytest.raises(RuntimeError, match="samples not evaluated exactly once"):
+        gmm.tod2map(out, tod, xg, plan, debug=True)
+    # An entry moved to another cell: its chunk now spans two cells.
+    plan = gmm.PointingPlan(pre, xg)
+    j = int(np.nonzero(cell[1:] != cell[:-1])[0][3])
+    _corrupt_entry(plan, dev, j, int((mt[j] & ~np.uint64((1 << 20) - 1)) | cell[j + 1]))
+    with pytest.raises(RuntimeError, match="spanning cells"):
+        plan.check_invariants()
+    # The untouched plan passes.
+    gmm.PointingPlan(pre, xg).check_invariants()
