@@ -545,41 +545,60 @@ __global__ void chunk_fill_kernel(const uint32_t *__restrict__ begin, const uint
 // X's own samples go left, so no block both gives and takes, and each sample has exactly one
 // evaluator; every entry of l computes these from the same 32 samples, so they agree. Bit q of
 // qmask[e] is set iff quad q (samples 4q..4q+3) holds a sample that Y evaluates; QM_TAKES_RIGHT
-// is R(l, Y). A line that lies in one cell gives all 8 quads to it. Also counts the multi-cell
-// lines (one zflag entry each). Eight consecutive threads handle the eight quads of one entry.
+// is R(l, Y). A line that lies in one cell gives all quads that hold a sample to it. Also counts
+// the multi-cell lines (one zflag entry each).
 template<bool Periodic>
 __global__ void __launch_bounds__(256)
 qmask_kernel(const uint64_t *__restrict__ mt, int64_t n, const float *__restrict__ xp, int64_t nsamp, int ny,
              int nx, uint16_t *__restrict__ qmask, uint32_t *__restrict__ nmlines)
 {
+    // Each wave takes 64 entries at a time, one per lane. An entry of a single-cell line owns
+    // every quad of its line that holds a sample: written directly. The entries of multi-cell
+    // lines are listed in LDS and handled eight at a time, eight lanes each (one quad per lane).
     __shared__ uint32_t s_nml;
+    __shared__ uint32_t s_list[256 / WAVE][WAVE];
     if (threadIdx.x == 0)
         s_nml = 0;
     __syncthreads();
-    const int lane = lane_id();
-    const int64_t total = 8 * n;
+    const int lane = lane_id(), wave = threadIdx.x / WAVE;
+    const int g = lane >> 3, q = lane & 7;
+    uint32_t *list = s_list[wave];
     // Line summary bits, OR-reduced over the entry's 32 samples.
     constexpr uint32_t HAS_L = 1, BAD_L = 2, HAS_Y = 4, BAD_Y = 8, HAS_R = 16, BAD_R = 32, TOUCH_L = 64,
                        TOUCH_LL = 128;
-    for (int64_t t0 = int64_t(blockIdx.x) * 256; t0 < total; t0 += int64_t(gridDim.x) * 256) {
-        const int64_t t = t0 + threadIdx.x;
-        const int64_t e = t >> 3;
-        const int q = int(t & 7);
-        bool zf = false, multi = false;
-        uint32_t icell = 0, sum = 0, own_y = 0, own_r = 0;     // own_*: per sample k, bit k
-        if (t < total) {
-            const uint64_t m = mt[e];
-            zf = (q == 0) && (m & ZFLAG);
-            multi = (m & MFLAG) != 0;
-            icell = uint32_t(m) & CELL_MASK;
-            const int cy = int(icell >> 10), cx = int(icell & 1023u);
-            const int64_t s0 = int64_t(uint32_t(m >> ICL_SHIFT) & ICL_MASK) * LINE + 4 * q;
+    const int64_t nwaves = int64_t(gridDim.x) * (256 / WAVE);
+    for (int64_t base = (int64_t(blockIdx.x) * (256 / WAVE) + wave) * WAVE; base < n; base += nwaves * WAVE) {
+        const int64_t e = base + lane;
+        const uint64_t m = (e < n) ? mt[e] : 0ull;
+        const bool multi = (e < n) && (m & MFLAG);
+        const uint64_t z = __ballot((e < n) && (m & ZFLAG));
+        if (lane == 0 && z)
+            atomicAdd(&s_nml, uint32_t(__popcll(z)));
+        if (e < n && !multi) {
             // Samples past the end of a partial last line do not exist.
-            const uint32_t nv = uint32_t(max<int64_t>(0, min<int64_t>(4, nsamp - s0)));
-            if (!multi)
-                own_y = (1u << nv) - 1u;
+            const int64_t s0 = int64_t(uint32_t(m >> ICL_SHIFT) & ICL_MASK) * LINE;
+            const int64_t nv = max<int64_t>(0, min<int64_t>(LINE, nsamp - s0));
+            qmask[e] = uint16_t((1u << uint32_t((nv + 3) / 4)) - 1u);
+        }
+        const uint64_t mb = __ballot(multi);
+        if (mb == 0)
+            continue;
+        if (multi)
+            list[__builtin_amdgcn_mbcnt_hi(uint32_t(mb >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mb), 0u))] = uint32_t(lane);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        const int nm = __popcll(mb);
+        for (int r0 = 0; r0 < nm; r0 += 8) {
+            const bool on = r0 + g < nm;
+            const int src = on ? int(list[r0 + g]) : 0;
+            const uint64_t em = __shfl(m, src, WAVE);
+            const int64_t ee = base + src;
+            uint32_t sum = 0, own_y = 0, own_r = 0;     // own_*: per sample k, bit k
+            const uint32_t icell = uint32_t(em) & CELL_MASK;
+            const int cy = int(icell >> 10), cx = int(icell & 1023u);
+            const int64_t s0 = int64_t(uint32_t(em >> ICL_SHIFT) & ICL_MASK) * LINE + 4 * q;
+            const uint32_t nv = on ? uint32_t(max<int64_t>(0, min<int64_t>(4, nsamp - s0))) : 0u;
             float yv[4], xv[4];
-            if (multi && nv > 0) {
+            if (nv > 0) {
                 // All loads first (indices clamped to the last sample), so that they overlap.
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
@@ -590,7 +609,7 @@ qmask_kernel(const uint64_t *__restrict__ mt, int64_t n, const float *__restrict
             }
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                if (!multi || uint32_t(k) >= nv)
+                if (uint32_t(k) >= nv)
                     continue;
                 Pixel<float> p;
                 locate_pixel(yv[k], xv[k], ny, nx, Periodic, p);
@@ -610,22 +629,22 @@ qmask_kernel(const uint64_t *__restrict__ mt, int64_t n, const float *__restrict
                     sum |= (d == -2 || d1 == -2) ? TOUCH_LL : 0u;
                 }
             }
+            // OR over the 8 lanes of the entry (aligned groups of 8 within the wave).
+            sum |= __shfl_xor(sum, 1, WAVE);
+            sum |= __shfl_xor(sum, 2, WAVE);
+            sum |= __shfl_xor(sum, 4, WAVE);
+            const bool elig_l = (sum & HAS_L) && !(sum & BAD_L) && (sum & TOUCH_LL);
+            const bool elig_y = (sum & HAS_Y) && !(sum & BAD_Y) && (sum & TOUCH_L);
+            const bool taken_y = elig_y && !elig_l;                  // R(l, Y - 1)
+            const bool takes_r = (sum & HAS_R) && !(sum & BAD_R) && !elig_y;   // R(l, Y)
+            const bool bit = (own_y != 0 && !taken_y) || (own_r != 0 && takes_r);
+            const uint64_t b = __ballot(bit);
+            if (on && q == 0)
+                qmask[ee] = uint16_t(((b >> (lane & ~7)) & 0xffu) | (takes_r ? QM_TAKES_RIGHT : 0u));
         }
-        // OR over the 8 lanes of the entry (aligned groups of 8 within the wave).
-        sum |= __shfl_xor(sum, 1, WAVE);
-        sum |= __shfl_xor(sum, 2, WAVE);
-        sum |= __shfl_xor(sum, 4, WAVE);
-        const bool elig_l = (sum & HAS_L) && !(sum & BAD_L) && (sum & TOUCH_LL);
-        const bool elig_y = (sum & HAS_Y) && !(sum & BAD_Y) && (sum & TOUCH_L);
-        const bool taken_y = elig_y && !elig_l;                  // R(l, Y - 1)
-        const bool takes_r = (sum & HAS_R) && !(sum & BAD_R) && !elig_y;   // R(l, Y)
-        const bool bit = (own_y != 0 && !taken_y) || (own_r != 0 && takes_r);
-        const uint64_t b = __ballot(bit);
-        if (t < total && q == 0)
-            qmask[e] = uint16_t(((b >> (lane & ~7)) & 0xffu) | (takes_r ? QM_TAKES_RIGHT : 0u));
-        const uint64_t z = __ballot(zf);
-        if (lane == 0 && z)
-            atomicAdd(&s_nml, uint32_t(__popcll(z)));
+        // The list is read before the next batch writes it (LDS operations of a wave complete
+        // in issue order; the fence keeps the compiler from reordering them).
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     }
     __syncthreads();
     if (threadIdx.x == 0 && s_nml)
@@ -949,7 +968,7 @@ void plan_schedule(const gmm_preplan &pp, const uint64_t *sorted_mt, const float
 
     // Owned-quad masks and the multi-cell line count.
     fill_u32_kernel<<<1, 64, 0, stream>>>(counts_gpu + 1, 1, 0u);
-    const int grid = int(std::min<int64_t>((8 * n + 255) / 256, 2048));
+    const int grid = int(std::min<int64_t>((n + 255) / 256, 2048));
     if (pp.periodic_xcoord)
         qmask_kernel<true><<<grid, 256, 0, stream>>>(sorted_mt, n, xp, pp.nsamp, int(pp.nypix_global),
                                                      int(pp.nxpix_global), qmask, counts_gpu + 1);
