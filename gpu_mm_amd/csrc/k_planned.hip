@@ -833,6 +833,15 @@ __device__ inline bool t2m_sample(unsigned long long *__restrict__ acc, float *_
     return owned;
 }
 
+// sin and cos of x0 + d from s0 = sin x0, c0 = cos x0 for |d| <= 1e-4, where cos d = 1 and
+// sin d = d are exact to float rounding (the dropped d^2/2 <= 5e-9 and d^3/6 are below half an
+// ulp of values near 1, and below the float rounding of the products otherwise): two FMAs.
+__device__ inline void angle_step_tiny(float s0, float c0, float d, float &sn, float &cs)
+{
+    sn = fmaf(d, c0, s0);
+    cs = fmaf(-d, s0, c0);
+}
+
 // sin and cos of x0 + d from s0 = sin x0, c0 = cos x0 for |d| <= 0.0185, where cos d = 1 - d^2/2
 // and sin d = d (1 - d^2/6) are exact to float rounding (the bound of the table's reduction).
 __device__ inline void angle_step(float s0, float c0, float d, float &sn, float &cs)
@@ -892,6 +901,7 @@ __device__ inline void t2m_quad_fixed(unsigned long long *__restrict__ acc, cons
     const float x0 = 2.0f * av[0];
     const float dmax = fmaxf(fmaxf(fabsf(2.0f * av[1] - x0), fabsf(2.0f * av[2] - x0)), fabsf(2.0f * av[3] - x0));
     const bool near = Small && !__any(!(dmax <= 0.0185f));
+    const bool tiny = near && !__any(!(dmax <= 1.0e-4f));
     if (ey - by <= 1 && ex - bx <= 1) {
         float sum[3][3][3];
 #pragma unroll
@@ -905,7 +915,9 @@ __device__ inline void t2m_quad_fixed(unsigned long long *__restrict__ acc, cons
         for (int k = 0; k < 4; k++) {
             float sn = s0, cs = c0;
             if (k > 0) {
-                if (near)
+                if (tiny)
+                    angle_step_tiny(s0, c0, 2.0f * av[k] - x0, sn, cs);
+                else if (near)
                     angle_step(s0, c0, 2.0f * av[k] - x0, sn, cs);
                 else
                     sincos2<Small>(tab, av[k], sn, cs);
