@@ -330,7 +330,8 @@ __device__ inline void quad_scale(const Gains &g, uint32_t s0, float (&sc)[4])
 
 // ------------------------------------------------------------------------------------------
 // Sample location for the operators: the same index and offset arithmetic as locate_pixel()
-// (so the kernels agree with the planner bit for bit), with the range checks deferred:
+// (so the kernels agree with the planner bit for bit; the x offset x - ix is taken as fract(x),
+// the same value for every x in range), with the range checks deferred:
 //   - y (valid range [0, ny-1]) and non-periodic x ([0, nx-2]): each lane keeps the largest bit
 //     pattern of coordinate + 0.0f (the addition turns -0.0 into +0.0). Read as unsigned
 //     integers, the bit patterns of the floats in [0, hi] are exactly those <= bits(hi): any
@@ -402,13 +403,13 @@ __device__ inline void locate_op(float y, float x, const Geom &g, Checks &c, int
         x = (x >= 0.0f) ? x : (x + g.fnx);
         x = (x <= g.fnx) ? x : (x - g.fnx);
         const int ix = int(clamp0(x, g.fnx));
-        dx = x - float(ix);
+        dx = __builtin_amdgcn_fractf(x);                  // x - ix for x in [0, nx]
         ix0 = (ix < g.nx) ? ix : 0;
     }
     else {
         c.ux = max(c.ux, __float_as_uint(x + 0.0f));
         const int ix = int(clamp0(x, g.fnx2));
-        dx = x - float(ix);
+        dx = __builtin_amdgcn_fractf(x);                  // x - ix for x in [0, nx - 2]
         ix0 = ix;
     }
 }
@@ -859,12 +860,14 @@ __device__ inline void angle_step(float s0, float c0, float d, float &sn, float 
 // added with one LDS atomic per touched pixel and plane (18 for a group that stays in one pixel
 // row, at most 27) instead of 12 per sample. A pixel's sum of at most four contributions, each
 // below 2^28 in magnitude, rounds to an int32. Other groups deposit sample by sample.
+// Returns a value that is NaN iff a value the block deposits for the group is NaN (an evaluated
+// sample with t, or t times a gain, NaN): the caller's check for the float fallback.
 template<bool Periodic, bool HasResp, bool Small, bool Debug = false>
-__device__ inline void t2m_quad_fixed(unsigned long long *__restrict__ acc, const float2 *__restrict__ tab,
-                                      const float4 &y4, const float4 &x4, const float4 &a4, const float4 &t4,
-                                      const Geom &g, int ybase, int xbase, unsigned lxlim, const float (&tr)[4],
-                                      const float (&pr)[4], Checks &chk, const Dbg &dbg = Dbg{}, int64_t samp0 = 0,
-                                      uint32_t nv = 4, bool count = false)
+__device__ inline float t2m_quad_fixed(unsigned long long *__restrict__ acc, const float2 *__restrict__ tab,
+                                       const float4 &y4, const float4 &x4, const float4 &a4, const float4 &t4,
+                                       const Geom &g, int ybase, int xbase, unsigned lxlim, const float (&tr)[4],
+                                       const float (&pr)[4], Checks &chk, const Dbg &dbg = Dbg{}, int64_t samp0 = 0,
+                                       uint32_t nv = 4, bool count = false)
 {
     const float yv[4] = { y4.x, y4.y, y4.z, y4.w }, xv[4] = { x4.x, x4.y, x4.z, x4.w };
     const float av[4] = { a4.x, a4.y, a4.z, a4.w }, tv[4] = { t4.x, t4.y, t4.z, t4.w };
@@ -892,7 +895,15 @@ __device__ inline void t2m_quad_fixed(unsigned long long *__restrict__ acc, cons
         }
     }
     if (ey < 0)
-        return;
+        return 0.0f;
+    // The block's share of each sample: its T value and its polarized (Q/U) value before the
+    // angle factor; zero for a sample the block does not evaluate.
+    float ts[4], ps[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        ts[k] = own[k] ? (HasResp ? tv[k] * tr[k] : tv[k]) : 0.0f;
+        ps[k] = HasResp ? (own[k] ? tv[k] * pr[k] : 0.0f) : ts[k];
+    }
     // sin and cos of 2 alpha: from the table for sample 0 and, when every lane's angles stay
     // within |2 alpha_k - 2 alpha_0| <= 0.0185, for the other samples by angle addition from
     // sample 0 (consecutive samples of one detector); otherwise from the table.
@@ -903,6 +914,74 @@ __device__ inline void t2m_quad_fixed(unsigned long long *__restrict__ acc, cons
     const bool near = Small && !__any(!(dmax <= 0.0185f));
     const bool tiny = near && !__any(!(dmax <= 1.0e-4f));
     if (ey - by <= 1 && ex - bx <= 1) {
+        // The window's rows 0..1 and columns 0..1 are always touched; row 2 when the evaluated
+        // samples span two rows (ey > by), column 2 likewise. Touched rows and columns lie inside
+        // the tile: ey + 1 <= 64 and ex + 1 <= lxlim <= HS - 1.
+        const bool row2 = ey > by, col2 = ex > bx;
+        const int o = by * HS + bx;
+        if constexpr (Debug)
+            dbg_tile(dbg, o, (row2 ? 2 : 1) * HS + (col2 ? 2 : 1));   // the window's touched extent
+        // Window weights of sample k: rows r[0..2] (times the sample's value) and columns c[0..2].
+        // The sample's corner rows are ly and ly + 1, i.e. window rows 0, 1 or 1, 2; likewise
+        // columns.
+        auto weights = [&](int k, float v, float (&r)[3], float (&c)[3]) {
+            const bool ry = ly[k] != by, rx = lx[k] != bx;
+            const float v0 = v * (1.0f - dy[k]), v1 = v * dy[k];
+            r[0] = ry ? 0.0f : v0;
+            r[1] = ry ? v0 : v1;
+            r[2] = ry ? v1 : 0.0f;
+            const float wx0 = 1.0f - dx[k], wx1 = dx[k];
+            c[0] = rx ? 0.0f : wx0;
+            c[1] = rx ? wx0 : wx1;
+            c[2] = rx ? wx1 : 0.0f;
+        };
+        if (!HasResp && tiny) {
+            // First order in the angle steps d_k = 2 alpha_k - 2 alpha_0 (|d_k| <= 1e-4, where
+            // angle_step_tiny() is exact to float rounding): cos 2 alpha_k = c0 - d_k s0 and
+            // sin 2 alpha_k = s0 + d_k c0, so each pixel's Q = c0 A - s0 B and U = s0 A + c0 B
+            // with A the pixel's T sum (sum of w t) and B the sum of w t d. Each pixel is summed
+            // where it is deposited, so untouched row 2 / column 2 pixels cost nothing.
+            float r[4][3], c[4][3], rd[4][3];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                weights(k, ts[k], r[k], c[k]);
+                const float d = 2.0f * av[k] - x0;            // d_0 = 0: not used
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+                    rd[k][i] = r[k][i] * d;
+            }
+            auto pix = [&](int i, int j) {
+                float a = 0.0f, b = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    a = fmaf(r[k][i], c[k][j], a);
+#pragma unroll
+                for (int k = 1; k < 4; k++)
+                    b = fmaf(rd[k][i], c[k][j], b);
+                const int q = o + i * HS + j;
+                dep_fixed(acc + q, a);
+                dep_fixed(acc + HPLANE + q, fmaf(c0, a, -(s0 * b)));
+                dep_fixed(acc + 2 * HPLANE + q, fmaf(s0, a, c0 * b));
+                return a;
+            };
+            pix(0, 0);
+            pix(0, 1);
+            // Every evaluated sample's value enters window row 1 with a factor (times any
+            // column weight, zero included), so this sum is NaN iff such a value is NaN.
+            const float probe = pix(1, 0);
+            pix(1, 1);
+            if (col2) {
+                pix(0, 2);
+                pix(1, 2);
+            }
+            if (row2) {
+                pix(2, 0);
+                pix(2, 1);
+                if (col2)
+                    pix(2, 2);
+            }
+            return probe;
+        }
         float sum[3][3][3];
 #pragma unroll
         for (int p = 0; p < 3; p++)
@@ -922,33 +1001,30 @@ __device__ inline void t2m_quad_fixed(unsigned long long *__restrict__ acc, cons
                 else
                     sincos2<Small>(tab, av[k], sn, cs);
             }
-            const float ts = HasResp ? tv[k] * tr[k] : tv[k];
-            const float ps = HasResp ? tv[k] * pr[k] : tv[k];
-            const float v[3] = { ts, ps * cs, ps * sn };
-            // Row weights at window rows 0..2 (the sample's rows are ly and ly + 1), likewise
-            // columns; a sample the block does not evaluate gets zero weights.
-            const bool ry = ly[k] != by, rx = lx[k] != bx;
-            const float wy0 = own[k] ? 1.0f - dy[k] : 0.0f, wy1 = own[k] ? dy[k] : 0.0f;
-            const float wx0 = 1.0f - dx[k], wx1 = dx[k];
-            const float r[3] = { ry ? 0.0f : wy0, ry ? wy0 : wy1, ry ? wy1 : 0.0f };
-            const float c[3] = { rx ? 0.0f : wx0, rx ? wx0 : wx1, rx ? wx1 : 0.0f };
+            float r[3], c[3], rp[3];
+            weights(k, ts[k], r, c);
+            if (HasResp) {
+                float cp[3];
+                weights(k, ps[k], rp, cp);
+            }
+            else {
 #pragma unroll
-            for (int p = 0; p < 3; p++)
+                for (int i = 0; i < 3; i++)
+                    rp[i] = r[i];
+            }
 #pragma unroll
-                for (int i = 0; i < 3; i++) {
-                    const float rv = r[i] * v[p];
+            for (int i = 0; i < 3; i++) {
+                const float q = rp[i] * cs, u = rp[i] * sn;
 #pragma unroll
-                    for (int j = 0; j < 3; j++)
-                        sum[p][i][j] = fmaf(rv, c[j], sum[p][i][j]);
+                for (int j = 0; j < 3; j++) {
+                    sum[0][i][j] = fmaf(r[i], c[j], sum[0][i][j]);
+                    sum[1][i][j] = fmaf(q, c[j], sum[1][i][j]);
+                    sum[2][i][j] = fmaf(u, c[j], sum[2][i][j]);
                 }
+            }
         }
-        // The window's rows 0..1 and columns 0..1 are always touched; row 2 when the evaluated
-        // samples span two rows (ey > by), column 2 likewise. Touched rows and columns lie inside
-        // the tile: ey + 1 <= 64 and ex + 1 <= lxlim <= HS - 1.
-        const bool row2 = ey > by, col2 = ex > bx;
-        const int o = by * HS + bx;
-        if constexpr (Debug)
-            dbg_tile(dbg, o, (row2 ? 2 : 1) * HS + (col2 ? 2 : 1));   // the window's touched extent
+        // As above: NaN iff a deposited value is NaN (the T and, with gains, the Q sum).
+        const float probe = HasResp ? sum[0][1][0] + sum[1][1][0] : sum[0][1][0];
 #pragma unroll
         for (int p = 0; p < 3; p++)
 #pragma unroll
@@ -975,26 +1051,26 @@ __device__ inline void t2m_quad_fixed(unsigned long long *__restrict__ acc, cons
                     dep_fixed(acc + p * HPLANE + o + 2 * HS + 2, sum[p][2][2]);
             }
         }
+        return probe;
     }
-    else {
+    float probe = 0.0f;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            if (!own[k])
-                continue;
-            float sn = s0, cs = c0;
-            if (k > 0) {
-                if (near)
-                    angle_step(s0, c0, 2.0f * av[k] - x0, sn, cs);
-                else
-                    sincos2<Small>(tab, av[k], sn, cs);
-            }
-            const float ts = HasResp ? tv[k] * tr[k] : tv[k];
-            const float ps = HasResp ? tv[k] * pr[k] : tv[k];
-            if constexpr (Debug)
-                dbg_tile(dbg, ly[k] * HS + lx[k]);
-            deposit4<DEP_FIXED>(acc, nullptr, ly[k] * HS + lx[k], dy[k], dx[k], ts, ps * cs, ps * sn);
+    for (int k = 0; k < 4; k++) {
+        if (!own[k])
+            continue;
+        float sn = s0, cs = c0;
+        if (k > 0) {
+            if (near)
+                angle_step(s0, c0, 2.0f * av[k] - x0, sn, cs);
+            else
+                sincos2<Small>(tab, av[k], sn, cs);
         }
+        probe += HasResp ? ts[k] + ps[k] : ts[k];
+        if constexpr (Debug)
+            dbg_tile(dbg, ly[k] * HS + lx[k]);
+        deposit4<DEP_FIXED>(acc, nullptr, ly[k] * HS + lx[k], dy[k], dx[k], ts[k], ps[k] * cs, ps[k] * sn);
     }
+    return probe;
 }
 
 // Largest |t| (times the gains) over the first 8 plan entries of [wb, we), for the calling
@@ -1124,7 +1200,9 @@ __device__ inline void t2m_entries(unsigned long long *__restrict__ acc, float *
                 t4.w *= sc[3];
             }
             if (finite) {
-                // NaN makes fmaxf skip the value: count it through the sum instead.
+                // NaN makes fmaxf skip the value: tsum catches it instead (t2m_quad_fixed()
+                // returns a value that is NaN iff a value it deposits is; an outlier group adds
+                // its samples).
                 float tmax;
                 if (HasResp) {
                     tmax = fmaxf(fmaxf(fabsf(t4.x * tr[0]), fabsf(t4.x * pr[0])),
@@ -1134,19 +1212,19 @@ __device__ inline void t2m_entries(unsigned long long *__restrict__ acc, float *
                 }
                 else
                     tmax = fmaxf(fmaxf(fabsf(t4.x), fabsf(t4.y)), fmaxf(fabsf(t4.z), fabsf(t4.w)));
-                tsum += (t4.x + t4.y) + (t4.z + t4.w);
                 t4.x *= scale;
                 t4.y *= scale;
                 t4.z *= scale;
                 t4.w *= scale;
                 if (tmax < limit) {
                     if (quad_small_angles(a4))
-                        t2m_quad_fixed<Periodic, HasResp, true, Debug>(acc, sct, y4, x4, a4, t4, g, ybase, xbase, lxlim, tr, pr, chk, dbg, s0, nv, count);
+                        tsum += t2m_quad_fixed<Periodic, HasResp, true, Debug>(acc, sct, y4, x4, a4, t4, g, ybase, xbase, lxlim, tr, pr, chk, dbg, s0, nv, count);
                     else
-                        t2m_quad_fixed<Periodic, HasResp, false, Debug>(acc, sct, y4, x4, a4, t4, g, ybase, xbase, lxlim, tr, pr, chk, dbg, s0, nv, count);
+                        tsum += t2m_quad_fixed<Periodic, HasResp, false, Debug>(acc, sct, y4, x4, a4, t4, g, ybase, xbase, lxlim, tr, pr, chk, dbg, s0, nv, count);
                 }
                 else {
                     // Only an outlier group can exceed the 64-bit range: mrun is tracked here.
+                    tsum += (t4.x + t4.y) + (t4.z + t4.w);
                     mrun = fmaxf(mrun, tmax);
                     nwide++;
                     bool ob[4];
