@@ -1273,10 +1273,21 @@ __device__ inline void t2m_clear(unsigned long long *__restrict__ acc, bool fini
         a2[i] = make_ulonglong2(0ull, 0ull);
 }
 
+// Map value of tile word k: in fixed point the int64 sum times the inverse scale (a power of
+// two), rounded once to float (the sum is exact in double for |sum| < 2^53, from its two 32-bit
+// halves; a larger sum, which only outlier deposits reach, is rounded twice); otherwise the
+// float tile's value.
+template<bool Finite>
 __device__ inline float t2m_value(const unsigned long long *__restrict__ acc, const float *__restrict__ facc, int k,
-                                  bool finite, float inv_scale)
+                                  double inv_scale)
 {
-    return finite ? float((long long)acc[k]) * inv_scale : facc[k];
+    if constexpr (Finite) {
+        const unsigned long long v = acc[k];
+        const double d = fma(double(int(uint32_t(v >> 32))), 4294967296.0, double(uint32_t(v)));
+        return float(d * inv_scale);
+    }
+    else
+        return facc[k];
 }
 
 // Touches the cell's map lines (one float per 32-byte sector) at block start, so that the
@@ -1303,22 +1314,25 @@ __device__ inline void t2m_warm_cell(const float *__restrict__ lmap, const HaloC
 // split cell is flushed one pixel per lane (each atomic instruction then touches few cache
 // lines); an unsplit one as float4 groups of 4 pixels per thread where the group allows it.
 // Halo row 64 and columns W .. W+ncol-1 go to the neighbouring cells, atomically; deposits into
-// absent cells are dropped.
-template<int NT>
-__device__ inline void t2m_flush(float *__restrict__ lmap, const unsigned long long *__restrict__ acc,
-                                 const float *__restrict__ facc, const HaloCells &h, bool shared, bool finite,
-                                 float inv_scale, int64_t ystride, int64_t polstride, int tid)
+// absent cells are dropped. Index decoding uses shifts only (16 slots per tile row for the
+// column ranges of at most 16 pixels; row 0 and row 64 take 16-pixel strides).
+template<int NT, bool Finite>
+__device__ inline void t2m_flush_t(float *__restrict__ lmap, const unsigned long long *__restrict__ acc,
+                                   const float *__restrict__ facc, const HaloCells &h, bool shared, double inv_scale,
+                                   int64_t ystride, int64_t polstride, int tid)
 {
+    auto val = [&](int k) { return t2m_value<Finite>(acc, facc, k, inv_scale); };
     const bool vec = !shared && ((reinterpret_cast<uintptr_t>(lmap) & 15) == 0);
     // Columns [0, catom) may receive the left-hand cell's halo deposits.
     const int catom = (h.cx >= 1) ? HALO_X : 1;
+    static_assert(HALO_X <= 16, "halo columns fit the 16 slots of a tile row");
     if (h.own >= 0 && !vec) {
         // Whole cell, one pixel per lane; atomic where another block may add too.
         for (int i = tid; i < 3 * CELL_NPIX; i += NT) {
             const int pl = i >> 12, r = (i >> 6) & 63, c = i & 63;
             if (c == h.w)
                 continue;
-            const float v = t2m_value(acc, facc, pl * HPLANE + r * HS + c, finite, inv_scale);
+            const float v = val(pl * HPLANE + r * HS + c);
             float *gp = lmap + h.own + pl * polstride + r * ystride + c;
             if (shared || r == 0 || c < catom)
                 unsafeAtomicAdd(gp, v);
@@ -1329,51 +1343,49 @@ __device__ inline void t2m_flush(float *__restrict__ lmap, const unsigned long l
     else if (h.own >= 0) {
         // Row 0 and columns [0, catom) of rows 1..63, which other blocks add to as well: one
         // pixel per lane, atomically, so that the lanes of an atomic instruction share few cache
-        // lines (the 12 columns of a row are one 48-byte run).
-        const int nat = 64 + 63 * catom;                  // pixels per plane
-        for (int i = tid; i < 3 * nat; i += NT) {
-            const int pl = i / nat, k = i - pl * nat;
-            int r = 0, c = k;
-            if (k >= 64) {
-                r = 1 + (k - 64) / catom;
-                c = (k - 64) - (r - 1) * catom;
+        // lines (the 12 columns of a row are one 48-byte run). Slot c of row 0 takes columns c,
+        // c + 16, c + 32, c + 48.
+        for (int i = tid; i < 3 * 64 * 16; i += NT) {
+            const int pl = i >> 10, r = (i >> 4) & 63, c = i & 15;
+            float *gp = lmap + h.own + pl * polstride + r * ystride;
+            const int kt = pl * HPLANE + r * HS;
+            if (r == 0) {
+#pragma unroll
+                for (int j = c; j < 64; j += 16)
+                    if (j != h.w)
+                        unsafeAtomicAdd(gp + j, val(kt + j));
             }
-            if (c == h.w)
-                continue;
-            unsafeAtomicAdd(lmap + h.own + pl * polstride + r * ystride + c,
-                            t2m_value(acc, facc, pl * HPLANE + r * HS + c, finite, inv_scale));
+            else if (c < catom && c != h.w)
+                unsafeAtomicAdd(gp + c, val(kt + c));
         }
         // Rows 1..63 from column catom on: groups of 4 pixels of a row, all planes, as float4
-        // read-modify-writes (all of a thread's loads issued first); a group that holds a column
-        // below catom or column W pixel by pixel.
+        // read-modify-writes. Every group's load is issued (all inside the cell, so the loads go
+        // out together); a group holding row 0, a column below catom or column W is done pixel
+        // by pixel.
         constexpr int NGRP = 3 * CELL_NPIX / 4;
-        constexpr int NG = (NGRP + NT - 1) / NT;
+        constexpr int NG = NGRP / NT;
+        static_assert(NG * NT == NGRP, "block size must divide the cell's float4 groups");
         float4 old[NG];
-        bool full[NG];
 #pragma unroll
         for (int k = 0; k < NG; k++) {
             const int g = tid + k * NT;
             const int pl = g >> 10, r = (g >> 4) & 63, c0 = (g & 15) << 2;
-            full[k] = (g < NGRP) && r != 0 && c0 >= catom && (h.w < c0 || h.w > c0 + 3);
-            if (full[k])
-                old[k] = *reinterpret_cast<const float4 *>(lmap + h.own + pl * polstride + r * ystride + c0);
+            old[k] = *reinterpret_cast<const float4 *>(lmap + h.own + pl * polstride + r * ystride + c0);
         }
 #pragma unroll
         for (int k = 0; k < NG; k++) {
             const int g = tid + k * NT;
-            if (g >= NGRP)
-                continue;
             const int pl = g >> 10, r = (g >> 4) & 63, c0 = (g & 15) << 2;
             if (r == 0 || c0 + 3 < catom)
                 continue;
             const int kt = pl * HPLANE + r * HS + c0;
             float *gp = lmap + h.own + pl * polstride + r * ystride + c0;
-            if (full[k]) {
+            if (c0 >= catom && (h.w < c0 || h.w > c0 + 3)) {
                 float4 v = old[k];
-                v.x += t2m_value(acc, facc, kt, finite, inv_scale);
-                v.y += t2m_value(acc, facc, kt + 1, finite, inv_scale);
-                v.z += t2m_value(acc, facc, kt + 2, finite, inv_scale);
-                v.w += t2m_value(acc, facc, kt + 3, finite, inv_scale);
+                v.x += val(kt);
+                v.y += val(kt + 1);
+                v.z += val(kt + 2);
+                v.w += val(kt + 3);
                 *reinterpret_cast<float4 *>(gp) = v;
             }
             else {
@@ -1382,35 +1394,42 @@ __device__ inline void t2m_flush(float *__restrict__ lmap, const unsigned long l
                     const int c = c0 + j;
                     if (c < catom || c == h.w)
                         continue;
-                    gp[j] += t2m_value(acc, facc, kt + j, finite, inv_scale);
+                    gp[j] += val(kt + j);
                 }
             }
         }
     }
-    // Halo row 64 (columns 0 .. W+ncol-1) and columns W .. W+ncol-1 of rows 0..63, to the
-    // neighbouring cells.
-    const int nrow = h.w + h.ncol, nper = nrow + 64 * h.ncol;
-    for (int i = tid; i < 3 * nper; i += NT) {
-        const int pl = i / nper;
-        const int jj = i - pl * nper;
-        int k;
-        int64_t dst;
-        if (jj < nrow) {
-            k = 64 * HS + jj;
-            if (jj < h.w)
-                dst = (h.down >= 0) ? h.down + jj : -1;
-            else
-                dst = (h.diag >= 0) ? h.diag + (jj - h.w) : -1;
+    // Halo row 64 (columns 0 .. W+ncol-1: slot c takes c, c + 16, ...) and columns W .. W+ncol-1
+    // of rows 0..63 (slot c < ncol), to the neighbouring cells.
+    const int nrow = h.w + h.ncol;
+    constexpr int PSLOTS = 65 * 16;
+    for (int i = tid; i < 3 * PSLOTS; i += NT) {
+        const int pl = i / PSLOTS;
+        const int rem = i - pl * PSLOTS;
+        const int r = rem >> 4, c = rem & 15;
+        const int kt = pl * HPLANE + r * HS;
+        if (r == 64) {
+            for (int j = c; j < nrow; j += 16) {
+                const int64_t dst = (j < h.w) ? ((h.down >= 0) ? h.down + j : int64_t(-1))
+                                              : ((h.diag >= 0) ? h.diag + (j - h.w) : int64_t(-1));
+                if (dst >= 0)
+                    unsafeAtomicAdd(lmap + dst + pl * polstride, val(kt + j));
+            }
         }
-        else {
-            const int r = (jj - nrow) / h.ncol, c = (jj - nrow) - r * h.ncol;
-            k = r * HS + h.w + c;
-            dst = (h.right >= 0) ? h.right + r * ystride + c : -1;
-        }
-        if (dst < 0)
-            continue;
-        unsafeAtomicAdd(lmap + dst + pl * polstride, t2m_value(acc, facc, pl * HPLANE + k, finite, inv_scale));
+        else if (c < h.ncol && h.right >= 0)
+            unsafeAtomicAdd(lmap + h.right + pl * polstride + r * ystride + c, val(kt + h.w + c));
     }
+}
+
+template<int NT>
+__device__ inline void t2m_flush(float *__restrict__ lmap, const unsigned long long *__restrict__ acc,
+                                 const float *__restrict__ facc, const HaloCells &h, bool shared, bool finite,
+                                 float inv_scale, int64_t ystride, int64_t polstride, int tid)
+{
+    if (finite)
+        t2m_flush_t<NT, true>(lmap, acc, facc, h, shared, double(inv_scale), ystride, polstride, tid);
+    else
+        t2m_flush_t<NT, false>(lmap, acc, facc, h, shared, double(inv_scale), ystride, polstride, tid);
 }
 
 // Block-shared words of t2m_accumulate(): the probe bound (float bits), and per round a redo
