@@ -854,14 +854,195 @@ __device__ inline void angle_step(float s0, float c0, float d, float &sn, float 
     cs = fmaf(c0, cd, -(s0 * sd));
 }
 
-// Fixed-point tod2map of one 4-sample group (t already scaled). When the (iy0, ix0) corners of
-// the group's evaluated samples span at most two rows and two columns, the group's bilinear
-// footprint lies in a 3x3 pixel window: its contributions are summed per pixel in registers and
-// added with one LDS atomic per touched pixel and plane (18 for a group that stays in one pixel
-// row, at most 27) instead of 12 per sample. A pixel's sum of at most four contributions, each
-// below 2^28 in magnitude, rounds to an int32. Other groups deposit sample by sample.
-// Returns a value that is NaN iff a value the block deposits for the group is NaN (an evaluated
-// sample with t, or t times a gain, NaN): the caller's check for the float fallback.
+// Per-sample state of a 4-sample group in fixed-point tod2map: tile-local corner (ly, lx), offsets
+// (dy, dx), whether the block evaluates the sample (own), its T value and its polarized value
+// before the angle factor (ts, ps: zero when not evaluated), and its angle.
+struct QuadSamples
+{
+    int ly[4], lx[4];
+    float dy[4], dx[4];
+    bool own[4];
+    float ts[4], ps[4], av[4];
+};
+
+// Row and column range of the evaluated samples among K0..K1-1 (ey < 0 when there are none).
+template<int K0, int K1>
+__device__ inline void quad_extent(const QuadSamples &q, int &by, int &bx, int &ey, int &ex)
+{
+    by = 64;
+    bx = 1 << 20;
+    ey = ex = -1;
+#pragma unroll
+    for (int k = K0; k < K1; k++) {
+        if (q.own[k]) {
+            by = min(by, q.ly[k]);
+            ey = max(ey, q.ly[k]);
+            bx = min(bx, q.lx[k]);
+            ex = max(ex, q.lx[k]);
+        }
+    }
+}
+
+// Deposits of samples K0..K1-1 whose corners span at most two rows and two columns (rows by..ey,
+// columns bx..ex): their bilinear footprint lies in a 3x3 pixel window, summed per pixel in
+// registers and added with one LDS atomic per touched pixel and plane (18 when the samples stay
+// in one pixel row, at most 27) instead of 12 per sample. A pixel's sum of at most four
+// contributions, each below 2^28 in magnitude, rounds to an int32. (s0, c0) are sin and cos of
+// x0 = 2 alpha_0 (sample 0 of the group); 'near' and 'tiny' tell that every angle step d_k =
+// 2 alpha_k - x0 of the wave's groups is at most 0.0185 or 1e-4. Returns a value that is NaN
+// iff a value the block deposits is NaN.
+template<bool HasResp, bool Small, bool Debug, int K0, int K1>
+__device__ inline float t2m_window(unsigned long long *__restrict__ acc, const float2 *__restrict__ tab,
+                                   const QuadSamples &q, int by, int bx, int ey, int ex, float s0, float c0, float x0,
+                                   bool near, bool tiny, const Dbg &dbg)
+{
+    // The window's rows 0..1 and columns 0..1 are always touched; row 2 when the evaluated
+    // samples span two rows (ey > by), column 2 likewise. Touched rows and columns lie inside
+    // the tile: ey + 1 <= 64 and ex + 1 <= lxlim <= HS - 1.
+    const bool row2 = ey > by, col2 = ex > bx;
+    const int o = by * HS + bx;
+    if constexpr (Debug)
+        dbg_tile(dbg, o, (row2 ? 2 : 1) * HS + (col2 ? 2 : 1));   // the window's touched extent
+    // Window weights of sample k: rows r[0..2] (times the sample's value) and columns c[0..2].
+    // The sample's corner rows are ly and ly + 1, i.e. window rows 0, 1 or 1, 2; likewise
+    // columns.
+    auto weights = [&](int k, float v, float (&r)[3], float (&c)[3]) {
+        const bool ry = q.ly[k] != by, rx = q.lx[k] != bx;
+        const float v0 = v * (1.0f - q.dy[k]), v1 = v * q.dy[k];
+        r[0] = ry ? 0.0f : v0;
+        r[1] = ry ? v0 : v1;
+        r[2] = ry ? v1 : 0.0f;
+        const float wx0 = 1.0f - q.dx[k], wx1 = q.dx[k];
+        c[0] = rx ? 0.0f : wx0;
+        c[1] = rx ? wx0 : wx1;
+        c[2] = rx ? wx1 : 0.0f;
+    };
+    if (!HasResp && tiny) {
+        // First order in the angle steps (|d_k| <= 1e-4, where angle_step_tiny() is exact to
+        // float rounding): cos 2 alpha_k = c0 - d_k s0 and sin 2 alpha_k = s0 + d_k c0, so each
+        // pixel's Q = c0 A - s0 B and U = s0 A + c0 B with A the pixel's T sum (sum of w t) and
+        // B the sum of w t d. Each pixel is summed where it is deposited, so untouched row 2 /
+        // column 2 pixels cost nothing.
+        float r[4][3], c[4][3], rd[4][3];
+#pragma unroll
+        for (int k = K0; k < K1; k++) {
+            weights(k, q.ts[k], r[k], c[k]);
+            const float d = 2.0f * q.av[k] - x0;              // d_0 = 0: not used
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                rd[k][i] = r[k][i] * d;
+        }
+        auto pix = [&](int i, int j) {
+            float a = 0.0f, b = 0.0f;
+#pragma unroll
+            for (int k = K0; k < K1; k++)
+                a = fmaf(r[k][i], c[k][j], a);
+#pragma unroll
+            for (int k = (K0 == 0 ? 1 : K0); k < K1; k++)
+                b = fmaf(rd[k][i], c[k][j], b);
+            const int w = o + i * HS + j;
+            dep_fixed(acc + w, a);
+            dep_fixed(acc + HPLANE + w, fmaf(c0, a, -(s0 * b)));
+            dep_fixed(acc + 2 * HPLANE + w, fmaf(s0, a, c0 * b));
+            return a;
+        };
+        pix(0, 0);
+        pix(0, 1);
+        // Every evaluated sample's value enters window row 1 with a factor (times any column
+        // weight, zero included), so this sum is NaN iff such a value is NaN.
+        const float probe = pix(1, 0);
+        pix(1, 1);
+        if (col2) {
+            pix(0, 2);
+            pix(1, 2);
+        }
+        if (row2) {
+            pix(2, 0);
+            pix(2, 1);
+            if (col2)
+                pix(2, 2);
+        }
+        return probe;
+    }
+    float sum[3][3][3];
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                sum[p][i][j] = 0.0f;
+#pragma unroll
+    for (int k = K0; k < K1; k++) {
+        float sn = s0, cs = c0;
+        if (k > 0) {
+            if (tiny)
+                angle_step_tiny(s0, c0, 2.0f * q.av[k] - x0, sn, cs);
+            else if (near)
+                angle_step(s0, c0, 2.0f * q.av[k] - x0, sn, cs);
+            else
+                sincos2<Small>(tab, q.av[k], sn, cs);
+        }
+        float r[3], c[3], rp[3];
+        weights(k, q.ts[k], r, c);
+        if (HasResp) {
+            float cp[3];
+            weights(k, q.ps[k], rp, cp);
+        }
+        else {
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                rp[i] = r[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const float qv = rp[i] * cs, uv = rp[i] * sn;
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                sum[0][i][j] = fmaf(r[i], c[j], sum[0][i][j]);
+                sum[1][i][j] = fmaf(qv, c[j], sum[1][i][j]);
+                sum[2][i][j] = fmaf(uv, c[j], sum[2][i][j]);
+            }
+        }
+    }
+    // As above: NaN iff a deposited value is NaN (the T and, with gains, the Q sum).
+    const float probe = HasResp ? sum[0][1][0] + sum[1][1][0] : sum[0][1][0];
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+                dep_fixed(acc + p * HPLANE + o + i * HS + j, sum[p][i][j]);
+    if (col2) {
+#pragma unroll
+        for (int p = 0; p < 3; p++)
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+                dep_fixed(acc + p * HPLANE + o + i * HS + 2, sum[p][i][2]);
+    }
+    if (row2) {
+#pragma unroll
+        for (int p = 0; p < 3; p++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+                dep_fixed(acc + p * HPLANE + o + 2 * HS + j, sum[p][2][j]);
+        if (col2) {
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+                dep_fixed(acc + p * HPLANE + o + 2 * HS + 2, sum[p][2][2]);
+        }
+    }
+    return probe;
+}
+
+// Fixed-point tod2map of one 4-sample group (t already scaled): one 3x3 window (t2m_window())
+// when, in every lane of the wave, the evaluated samples' corners span at most two rows and two
+// columns (scan speeds below ~0.33 pixel per sample); otherwise a window per pair of samples
+// (samples 0-1 and 2-3, which fit one below 1 pixel per sample), and sample by sample for a
+// lane whose pair does not fit. The choice is per wave, so a wave runs one of the forms. Returns
+// a value that is NaN iff a value the block deposits for the group is NaN (an evaluated sample
+// with t, or t times a gain, NaN): the caller's check for the float fallback.
 template<bool Periodic, bool HasResp, bool Small, bool Debug = false>
 __device__ inline float t2m_quad_fixed(unsigned long long *__restrict__ acc, const float2 *__restrict__ tab,
                                        const float4 &y4, const float4 &x4, const float4 &a4, const float4 &t4,
@@ -870,27 +1051,29 @@ __device__ inline float t2m_quad_fixed(unsigned long long *__restrict__ acc, con
                                        uint32_t nv = 4, bool count = false)
 {
     const float yv[4] = { y4.x, y4.y, y4.z, y4.w }, xv[4] = { x4.x, x4.y, x4.z, x4.w };
-    const float av[4] = { a4.x, a4.y, a4.z, a4.w }, tv[4] = { t4.x, t4.y, t4.z, t4.w };
-    int ly[4], lx[4];
-    float dy[4], dx[4];
-    bool own[4];
+    const float tv[4] = { t4.x, t4.y, t4.z, t4.w };
+    QuadSamples q;
+    q.av[0] = a4.x;
+    q.av[1] = a4.y;
+    q.av[2] = a4.z;
+    q.av[3] = a4.w;
     int by = 64, bx = 1 << 20, ey = -1, ex = -1;          // row and column range of evaluated samples
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         int iy0, ix0;
-        locate_op<Periodic>(yv[k], xv[k], g, chk, iy0, ix0, dy[k], dx[k]);
-        ly[k] = iy0 - ybase;
-        lx[k] = ix0 - xbase;
-        own[k] = (unsigned(ly[k]) < 64u) && (unsigned(lx[k]) < lxlim);
-        if (own[k]) {
-            by = min(by, ly[k]);
-            ey = max(ey, ly[k]);
-            bx = min(bx, lx[k]);
-            ex = max(ex, lx[k]);
+        locate_op<Periodic>(yv[k], xv[k], g, chk, iy0, ix0, q.dy[k], q.dx[k]);
+        q.ly[k] = iy0 - ybase;
+        q.lx[k] = ix0 - xbase;
+        q.own[k] = (unsigned(q.ly[k]) < 64u) && (unsigned(q.lx[k]) < lxlim);
+        if (q.own[k]) {
+            by = min(by, q.ly[k]);
+            ey = max(ey, q.ly[k]);
+            bx = min(bx, q.lx[k]);
+            ex = max(ex, q.lx[k]);
         }
         if constexpr (Debug) {
             // Samples past the end of the TOD read as zeros: they deposit nothing.
-            if (count && own[k] && uint32_t(k) < nv)
+            if (count && q.own[k] && uint32_t(k) < nv)
                 atomicAdd(dbg.sample_evals + samp0 + k, 1u);
         }
     }
@@ -898,177 +1081,50 @@ __device__ inline float t2m_quad_fixed(unsigned long long *__restrict__ acc, con
         return 0.0f;
     // The block's share of each sample: its T value and its polarized (Q/U) value before the
     // angle factor; zero for a sample the block does not evaluate.
-    float ts[4], ps[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        ts[k] = own[k] ? (HasResp ? tv[k] * tr[k] : tv[k]) : 0.0f;
-        ps[k] = HasResp ? (own[k] ? tv[k] * pr[k] : 0.0f) : ts[k];
+        q.ts[k] = q.own[k] ? (HasResp ? tv[k] * tr[k] : tv[k]) : 0.0f;
+        q.ps[k] = HasResp ? (q.own[k] ? tv[k] * pr[k] : 0.0f) : q.ts[k];
     }
     // sin and cos of 2 alpha: from the table for sample 0 and, when every lane's angles stay
     // within |2 alpha_k - 2 alpha_0| <= 0.0185, for the other samples by angle addition from
     // sample 0 (consecutive samples of one detector); otherwise from the table.
     float s0, c0;
-    sincos2<Small>(tab, av[0], s0, c0);
-    const float x0 = 2.0f * av[0];
-    const float dmax = fmaxf(fmaxf(fabsf(2.0f * av[1] - x0), fabsf(2.0f * av[2] - x0)), fabsf(2.0f * av[3] - x0));
+    sincos2<Small>(tab, q.av[0], s0, c0);
+    const float x0 = 2.0f * q.av[0];
+    const float dmax = fmaxf(fmaxf(fabsf(2.0f * q.av[1] - x0), fabsf(2.0f * q.av[2] - x0)), fabsf(2.0f * q.av[3] - x0));
     const bool near = Small && !__any(!(dmax <= 0.0185f));
     const bool tiny = near && !__any(!(dmax <= 1.0e-4f));
-    if (ey - by <= 1 && ex - bx <= 1) {
-        // The window's rows 0..1 and columns 0..1 are always touched; row 2 when the evaluated
-        // samples span two rows (ey > by), column 2 likewise. Touched rows and columns lie inside
-        // the tile: ey + 1 <= 64 and ex + 1 <= lxlim <= HS - 1.
-        const bool row2 = ey > by, col2 = ex > bx;
-        const int o = by * HS + bx;
-        if constexpr (Debug)
-            dbg_tile(dbg, o, (row2 ? 2 : 1) * HS + (col2 ? 2 : 1));   // the window's touched extent
-        // Window weights of sample k: rows r[0..2] (times the sample's value) and columns c[0..2].
-        // The sample's corner rows are ly and ly + 1, i.e. window rows 0, 1 or 1, 2; likewise
-        // columns.
-        auto weights = [&](int k, float v, float (&r)[3], float (&c)[3]) {
-            const bool ry = ly[k] != by, rx = lx[k] != bx;
-            const float v0 = v * (1.0f - dy[k]), v1 = v * dy[k];
-            r[0] = ry ? 0.0f : v0;
-            r[1] = ry ? v0 : v1;
-            r[2] = ry ? v1 : 0.0f;
-            const float wx0 = 1.0f - dx[k], wx1 = dx[k];
-            c[0] = rx ? 0.0f : wx0;
-            c[1] = rx ? wx0 : wx1;
-            c[2] = rx ? wx1 : 0.0f;
-        };
-        if (!HasResp && tiny) {
-            // First order in the angle steps d_k = 2 alpha_k - 2 alpha_0 (|d_k| <= 1e-4, where
-            // angle_step_tiny() is exact to float rounding): cos 2 alpha_k = c0 - d_k s0 and
-            // sin 2 alpha_k = s0 + d_k c0, so each pixel's Q = c0 A - s0 B and U = s0 A + c0 B
-            // with A the pixel's T sum (sum of w t) and B the sum of w t d. Each pixel is summed
-            // where it is deposited, so untouched row 2 / column 2 pixels cost nothing.
-            float r[4][3], c[4][3], rd[4][3];
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                weights(k, ts[k], r[k], c[k]);
-                const float d = 2.0f * av[k] - x0;            // d_0 = 0: not used
-#pragma unroll
-                for (int i = 0; i < 3; i++)
-                    rd[k][i] = r[k][i] * d;
-            }
-            auto pix = [&](int i, int j) {
-                float a = 0.0f, b = 0.0f;
-#pragma unroll
-                for (int k = 0; k < 4; k++)
-                    a = fmaf(r[k][i], c[k][j], a);
-#pragma unroll
-                for (int k = 1; k < 4; k++)
-                    b = fmaf(rd[k][i], c[k][j], b);
-                const int q = o + i * HS + j;
-                dep_fixed(acc + q, a);
-                dep_fixed(acc + HPLANE + q, fmaf(c0, a, -(s0 * b)));
-                dep_fixed(acc + 2 * HPLANE + q, fmaf(s0, a, c0 * b));
-                return a;
-            };
-            pix(0, 0);
-            pix(0, 1);
-            // Every evaluated sample's value enters window row 1 with a factor (times any
-            // column weight, zero included), so this sum is NaN iff such a value is NaN.
-            const float probe = pix(1, 0);
-            pix(1, 1);
-            if (col2) {
-                pix(0, 2);
-                pix(1, 2);
-            }
-            if (row2) {
-                pix(2, 0);
-                pix(2, 1);
-                if (col2)
-                    pix(2, 2);
-            }
-            return probe;
-        }
-        float sum[3][3][3];
-#pragma unroll
-        for (int p = 0; p < 3; p++)
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++)
-                    sum[p][i][j] = 0.0f;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            float sn = s0, cs = c0;
-            if (k > 0) {
-                if (tiny)
-                    angle_step_tiny(s0, c0, 2.0f * av[k] - x0, sn, cs);
-                else if (near)
-                    angle_step(s0, c0, 2.0f * av[k] - x0, sn, cs);
-                else
-                    sincos2<Small>(tab, av[k], sn, cs);
-            }
-            float r[3], c[3], rp[3];
-            weights(k, ts[k], r, c);
-            if (HasResp) {
-                float cp[3];
-                weights(k, ps[k], rp, cp);
-            }
-            else {
-#pragma unroll
-                for (int i = 0; i < 3; i++)
-                    rp[i] = r[i];
-            }
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                const float q = rp[i] * cs, u = rp[i] * sn;
-#pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    sum[0][i][j] = fmaf(r[i], c[j], sum[0][i][j]);
-                    sum[1][i][j] = fmaf(q, c[j], sum[1][i][j]);
-                    sum[2][i][j] = fmaf(u, c[j], sum[2][i][j]);
-                }
-            }
-        }
-        // As above: NaN iff a deposited value is NaN (the T and, with gains, the Q sum).
-        const float probe = HasResp ? sum[0][1][0] + sum[1][1][0] : sum[0][1][0];
-#pragma unroll
-        for (int p = 0; p < 3; p++)
-#pragma unroll
-            for (int i = 0; i < 2; i++)
-#pragma unroll
-                for (int j = 0; j < 2; j++)
-                    dep_fixed(acc + p * HPLANE + o + i * HS + j, sum[p][i][j]);
-        if (col2) {
-#pragma unroll
-            for (int p = 0; p < 3; p++)
-#pragma unroll
-                for (int i = 0; i < 2; i++)
-                    dep_fixed(acc + p * HPLANE + o + i * HS + 2, sum[p][i][2]);
-        }
-        if (row2) {
-#pragma unroll
-            for (int p = 0; p < 3; p++)
-#pragma unroll
-                for (int j = 0; j < 2; j++)
-                    dep_fixed(acc + p * HPLANE + o + 2 * HS + j, sum[p][2][j]);
-            if (col2) {
-#pragma unroll
-                for (int p = 0; p < 3; p++)
-                    dep_fixed(acc + p * HPLANE + o + 2 * HS + 2, sum[p][2][2]);
-            }
-        }
+    if (!__any(!(ey - by <= 1 && ex - bx <= 1)))
+        return t2m_window<HasResp, Small, Debug, 0, 4>(acc, tab, q, by, bx, ey, ex, s0, c0, x0, near, tiny, dbg);
+    int by1, bx1, ey1, ex1;
+    quad_extent<0, 2>(q, by, bx, ey, ex);
+    quad_extent<2, 4>(q, by1, bx1, ey1, ex1);
+    const bool fit0 = ey - by <= 1 && ex - bx <= 1, fit1 = ey1 - by1 <= 1 && ex1 - bx1 <= 1;
+    float probe = 0.0f;
+    if (fit0 && fit1) {
+        if (ey >= 0)
+            probe += t2m_window<HasResp, Small, Debug, 0, 2>(acc, tab, q, by, bx, ey, ex, s0, c0, x0, near, tiny, dbg);
+        if (ey1 >= 0)
+            probe += t2m_window<HasResp, Small, Debug, 2, 4>(acc, tab, q, by1, bx1, ey1, ex1, s0, c0, x0, near, tiny, dbg);
         return probe;
     }
-    float probe = 0.0f;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        if (!own[k])
+        if (!q.own[k])
             continue;
         float sn = s0, cs = c0;
         if (k > 0) {
             if (near)
-                angle_step(s0, c0, 2.0f * av[k] - x0, sn, cs);
+                angle_step(s0, c0, 2.0f * q.av[k] - x0, sn, cs);
             else
-                sincos2<Small>(tab, av[k], sn, cs);
+                sincos2<Small>(tab, q.av[k], sn, cs);
         }
-        probe += HasResp ? ts[k] + ps[k] : ts[k];
+        probe += HasResp ? q.ts[k] + q.ps[k] : q.ts[k];
         if constexpr (Debug)
-            dbg_tile(dbg, ly[k] * HS + lx[k]);
-        deposit4<DEP_FIXED>(acc, nullptr, ly[k] * HS + lx[k], dy[k], dx[k], ts[k], ps[k] * cs, ps[k] * sn);
+            dbg_tile(dbg, q.ly[k] * HS + q.lx[k]);
+        deposit4<DEP_FIXED>(acc, nullptr, q.ly[k] * HS + q.lx[k], q.dy[k], q.dx[k], q.ts[k], q.ps[k] * cs,
+                            q.ps[k] * sn);
     }
     return probe;
 }
