@@ -88,6 +88,21 @@ def form_buffers(torch, gmm, form, w, lmap, plan):
     return dict(tod=t16, out=o16, sc_in=sc, sc_out=osc)
 
 
+def settle(torch, gmm, bufs, lmap, out_map, xp, plan, seconds):
+    """Untimed FP32 steps for 'seconds': GPU clocks ramp up under sustained load (kernel durations
+    of a cold start fall by ~10% over the first second), and every form should be timed on an
+    equally warm GPU. Returns the number of steps run."""
+    n = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(8):
+            gmm.map2tod(bufs["out"], lmap, xp, plan)
+            gmm.tod2map(out_map, bufs["tod"], xp, plan)
+        torch.cuda.synchronize()
+        n += 8
+    return n
+
+
 def run_form(torch, gmm, bufs, lmap, out_map, xp, plan, steps, warmup):
     def step():
         gmm.map2tod(bufs["out"], lmap, xp, plan, tod_scale=bufs["sc_out"])
@@ -533,6 +548,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--settle", type=float, default=2.0,
+                    help="seconds of untimed FP32 steps before the first timed form (GPU clocks ramp up under "
+                         "load over about a second; without this the first form is timed on a colder GPU)")
     ap.add_argument("--ndet", type=int, default=2048)
     ap.add_argument("--nt", type=int, default=262144)
     ap.add_argument("--forms", default="f32,f16,bf16")
@@ -567,6 +585,11 @@ def main():
     plan, xp = w["plan"], w["xp"]
     gmm.set_errflag_check(False)
 
+    if args.settle > 0:
+        b32 = form_buffers(torch, gmm, "f32", w, lmap, plan)
+        n = settle(torch, gmm, b32, lmap, out_map, xp, plan, args.settle)
+        del b32
+        log(f"[bench] settle: {n} untimed FP32 steps in {args.settle:.1f} s")
     res = {}
     for form in forms:
         bufs = form_buffers(torch, gmm, form, w, lmap, plan)
