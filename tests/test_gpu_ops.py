@@ -50,6 +50,24 @@ def make_instance(kind, seed):
             xp[1, d] = (t * (1.7 + 0.3 * d)) % (2 * nx) - nx / 2
             xp[2, d] = rng.uniform(0, np.pi) + 0.01 * t
         return xp, ny, nx, True
+    if kind == "mixed":
+        # Detectors at different scan speeds and angle rates, so that the waves of one cell mix
+        # lanes whose 4-sample groups fit one 3x3 tod2map window (slow), fit one window per pair
+        # of samples (0.35-0.9 px/sample) or neither (above 1 px/sample), with angle steps of the
+        # first-order, second-order and table regimes.
+        ndet, nt, ny, nx = 12, 32 * 256, 256, 512
+        t = np.arange(nt, dtype=np.float64)
+        speeds = [0.05, 0.2, 0.3, 0.4, 0.55, 0.7, 0.9, 1.3, 0.12, 0.6, 2.5, 0.33]
+        xp = np.empty((3, ndet, nt), F32)
+        for d in range(ndet):
+            v, phi = speeds[d], rng.uniform(0, 2 * np.pi)
+            u = (v * t) % 360.0
+            u = np.where(u < 180.0, u, 360.0 - u) - 90.0         # back and forth over 180 px
+            xp[0, d] = 128 + 0.5 * u * np.sin(phi) + rng.uniform(-2, 2)
+            xp[1, d] = 256 + u * np.cos(phi) + rng.uniform(-2, 2)
+            rate = (1e-6, 1e-3, 0.0)[d % 3]
+            xp[2, d] = rng.uniform(0, np.pi) + rate * t if d % 3 < 2 else rng.uniform(-10, 10, nt)
+        return xp, ny, nx, False
     if kind == "toy":
         from gpu_mm_amd import ToyPointing
         tp = ToyPointing.make_random(1 << 20, noisy=False, rng=rng)
@@ -58,7 +76,7 @@ def make_instance(kind, seed):
     raise ValueError(kind)
 
 
-INSTANCES = [("cfg0", 0), ("fast_periodic", 1), ("toy", 2), ("toy", 3)]
+INSTANCES = [("cfg0", 0), ("fast_periodic", 1), ("toy", 2), ("toy", 3), ("mixed", 4)]
 
 
 @pytest.fixture(scope="module", params=INSTANCES, ids=lambda p: f"{p[0]}-{p[1]}")
