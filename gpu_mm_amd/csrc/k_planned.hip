@@ -1195,8 +1195,8 @@ __device__ inline uint32_t wave_probe_bound(float m, bool act)
 // fixed point with scale 'scale' when 'finite', else float atomics. In fixed point a group whose
 // largest |t| (times the gains) is below 'limit' takes the 32-bit path, any other group (an
 // outlier) deposits sample by sample in 64-bit fixed point and counts in nwide. Tracks per lane
-// the largest |t| of its outlier groups (mrun) and a sum that is NaN iff some deposited t is NaN
-// (tsum).
+// the largest |t| of its outlier groups (mrun) and a sum that is NaN whenever a value it deposits
+// is NaN (tsum; an outlier group adds all of its samples).
 template<bool Periodic, bool Vec, bool HasResp, typename TT, bool Debug = false>
 __device__ inline void t2m_entries(unsigned long long *__restrict__ acc, float *__restrict__ facc,
                                    const TT *__restrict__ tod, const float *__restrict__ xp,
@@ -1536,7 +1536,7 @@ __device__ inline bool t2m_accumulate(unsigned long long *__restrict__ acc, uint
         const float limit = __uint_as_float(uint32_t(e + 128) << 23);   // 2^(e+1)
         const uint32_t wide_limit = uint32_t(min(e + 128 + WIDE_BITS, 255)) << 23;
         float mrun = 0.0f;                               // largest |t| of this lane's outlier groups
-        float tsum = 0.0f;                               // NaN iff some deposited t is NaN
+        float tsum = 0.0f;                               // NaN if some deposited value is NaN
         uint32_t nwide = 0;                              // outlier groups of this lane
         Checks chk;
         run(finite, scale, limit, mrun, tsum, nwide, chk, pass == 0);
