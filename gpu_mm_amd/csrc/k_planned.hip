@@ -1490,7 +1490,8 @@ __device__ inline void t2m_flush(float *__restrict__ lmap, const unsigned long l
 
 // Block-shared words of t2m_accumulate(): the probe bound (float bits), and per round a redo
 // word (largest offending |t|, as float bits) and a count of outlier groups.
-enum { W_MAX = 0, W_REDO = 1, W_WIDE = 4, T2M_WORDS = 7 };
+// W_CLAIM: per round, the next entry batch of the block's waves.
+enum { W_MAX = 0, W_REDO = 1, W_WIDE = 4, W_CLAIM = 7, T2M_WORDS = 10 };
 
 // Outliers: the 64-bit path takes scaled values up to 2^WIDE_BITS times the 32-bit range (|v| <
 // 2^52), so a pixel's int64 sum stays exact for the first 2^11 deposits at the top of that range.
@@ -1616,13 +1617,29 @@ planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, con
     t2m_warm_cell<T2M_THREADS>(lmap, h, ystride, polstride, threadIdx.x);
     __syncthreads();
 
+    // The waves take the chunk's entries in batches of 64 from a block counter (one per round), so
+    // that they finish together: entries differ in cost (owned groups, window forms), and an even
+    // static split left the block waiting on its slowest wave (measured: 3.4% of the kernel).
     float inv_scale;
+    int round = 0;
     const bool finite = t2m_accumulate<T2M_THREADS>(
         acc, s_words, ch.end - ch.begin, g,
         [&](bool fin, float scale, float limit, float &mrun, float &tsum, uint32_t &nwide, Checks &chk, bool first) {
-            t2m_entries<Periodic, Vec, HasResp, TT, Debug>(acc, facc, tod, xp, plan_mt, qmask, nsamp, wb, we, sct, g,
-                                                           ybase, xbase, gn, fin, scale, limit, mrun, tsum, nwide, chk,
-                                                           err, dbg, first);
+            uint32_t *claim = s_words + W_CLAIM + min(round, 2);
+            round++;
+            const uint32_t n = ch.end - ch.begin;
+            for (;;) {
+                uint32_t b = 0;
+                if (lane == 0)
+                    b = atomicAdd(claim, 64u);
+                b = __builtin_amdgcn_readfirstlane(b);
+                if (b >= n)
+                    break;
+                t2m_entries<Periodic, Vec, HasResp, TT, Debug>(acc, facc, tod, xp, plan_mt, qmask, nsamp, ch.begin + b,
+                                                               ch.begin + min(b + 64u, n), sct, g, ybase, xbase, gn,
+                                                               fin, scale, limit, mrun, tsum, nwide, chk, err, dbg,
+                                                               first);
+            }
         },
         inv_scale, err, threadIdx.x);
     t2m_flush<T2M_THREADS>(lmap, acc, facc, h, (ch.flags & CHUNK_SHARED) != 0, finite, inv_scale, ystride,
@@ -1808,17 +1825,30 @@ batched_tod2map_kernel(float *__restrict__ lmap, const BatchTod *__restrict__ to
         t2m_warm_cell<T2M_THREADS>(lmap, h, ba.ystride, ba.polstride, opaque_tid());
         __syncthreads();
 
+        // Entries in batches of 64 from a block counter per round, as in the plain kernel.
         float inv_scale;
+        int round = 0;
         const bool finite = t2m_accumulate<T2M_THREADS>(
             acc, s_words, it.nent, g,
             [&](bool fin, float scale, float limit, float &mrun, float &tsum, uint32_t &nwide, Checks &chk, bool) {
-                batch_wave_segments(segs, it, wb, we, [&](const BatchSeg &sg, uint32_t b, uint32_t e) {
-                    const BatchTod td = tods[sg.tod];
-                    t2m_entries<Periodic, true, HasResp, TT>(acc, facc, static_cast<const TT *>(td.tod), td.xp,
-                                                           td.plan_mt, td.qmask, td.nsamp, b, e, sct, g, ybase, xbase,
-                                                           batch_gains(td), fin, scale, limit, mrun, tsum, nwide, chk,
-                                                           err);
-                });
+                uint32_t *claim = s_words + W_CLAIM + min(round, 2);
+                round++;
+                for (;;) {
+                    uint32_t b0 = 0;
+                    if (lane == 0)
+                        b0 = atomicAdd(claim, 64u);
+                    b0 = __builtin_amdgcn_readfirstlane(b0);
+                    if (b0 >= it.nent)
+                        break;
+                    batch_wave_segments(segs, it, b0, min(b0 + 64u, it.nent),
+                                        [&](const BatchSeg &sg, uint32_t b, uint32_t e) {
+                        const BatchTod td = tods[sg.tod];
+                        t2m_entries<Periodic, true, HasResp, TT>(acc, facc, static_cast<const TT *>(td.tod), td.xp,
+                                                               td.plan_mt, td.qmask, td.nsamp, b, e, sct, g, ybase,
+                                                               xbase, batch_gains(td), fin, scale, limit, mrun, tsum,
+                                                               nwide, chk, err);
+                    });
+                }
             },
             inv_scale, err, opaque_tid());
         t2m_flush<T2M_THREADS>(lmap, acc, facc, h, (it.flags & CHUNK_SHARED) != 0, finite, inv_scale, ba.ystride,
