@@ -719,6 +719,17 @@ __device__ inline void m2t_entries(TT *__restrict__ tod, const float *__restrict
     }
 }
 
+// Chunks of at least this many entries take map2tod's dynamic entry batches.
+constexpr uint32_t M2T_DYNAMIC_MIN = 2048;
+
+// Entries per claim from a block's shared entry counter: 64, or fewer for a small work unit so
+// that every wave of the block gets a share (a multiple of 8: a wave takes 8 entries per step).
+__device__ inline uint32_t claim_batch(uint32_t n, uint32_t nwaves)
+{
+    const uint32_t per = (n + nwaves - 1) / nwaves;
+    return min(64u, max(8u, (per + 7u) & ~7u));
+}
+
 template<bool Periodic, bool Vec, bool HasResp, typename TT, bool Debug = false>
 __global__ void __launch_bounds__(M2T_THREADS, 4)
 planned_map2tod_kernel(TT *__restrict__ tod, const float *__restrict__ lmap, const float *__restrict__ xp,
@@ -729,24 +740,45 @@ planned_map2tod_kernel(TT *__restrict__ tod, const float *__restrict__ lmap, con
 {
     __shared__ float4 tile[HPLANE];                       // 77 KiB
     __shared__ float2 sct[SCT];
+    __shared__ uint32_t s_claim;
 
     const Chunk ch = chunks[xcd_chunk(blockIdx.x, gridDim.x)];
     const HaloCells h = halo_cells(ch.icell, cell_offsets, nycells, nxcells, nx, Periodic);
     uint32_t err = (h.own < 0 && !partial) ? ERR_NOT_IN_PIXELIZATION : 0u;
 
+    if (threadIdx.x == 0)
+        s_claim = 0;
     fill_sincos_table(sct, M2T_THREADS);
     stage_halo_tile<M2T_THREADS>(tile, lmap, h, ystride, polstride, threadIdx.x);
     __syncthreads();
 
     const Geom g = make_geom(ny, nx, Periodic);
     Checks chk;
-    const int wave = threadIdx.x >> 6;
     const int ybase = int(ch.icell >> 10) << 6, xbase = int(ch.icell & 1023u) << 6;
 
-    uint32_t wb, we;
-    wave_slice(ch, wave, M2T_THREADS / WAVE, wb, we);
-    m2t_entries<Periodic, Vec, HasResp, TT, Debug>(tod, xp, plan_mt, qmask, nsamp, wb, we, tile, sct, g, ybase, xbase,
-                                                   accum, gn, chk, err, dbg);
+    // A large chunk's entries go to the waves in batches of 64 from a block counter, so that the
+    // waves finish together (0.9% faster on the 2048x262144 raster); a small chunk's are split
+    // evenly (with the counter, per-TOD calls on TODs of a few hundred detectors were 2% slower).
+    const uint32_t n = ch.end - ch.begin;
+    if (n < M2T_DYNAMIC_MIN) {
+        uint32_t wb, we;
+        wave_slice(ch, int(threadIdx.x >> 6), M2T_THREADS / WAVE, wb, we);
+        m2t_entries<Periodic, Vec, HasResp, TT, Debug>(tod, xp, plan_mt, qmask, nsamp, wb, we, tile, sct, g, ybase,
+                                                       xbase, accum, gn, chk, err, dbg);
+    }
+    else {
+        for (;;) {
+            uint32_t b = 0;
+            if ((threadIdx.x & 63) == 0)
+                b = atomicAdd(&s_claim, 64u);
+            b = __builtin_amdgcn_readfirstlane(b);
+            if (b >= n)
+                break;
+            m2t_entries<Periodic, Vec, HasResp, TT, Debug>(tod, xp, plan_mt, qmask, nsamp, ch.begin + b,
+                                                           ch.begin + min(b + 64u, n), tile, sct, g, ybase, xbase,
+                                                           accum, gn, chk, err, dbg);
+        }
+    }
     report_err(err | chk.bits(g), err_word);
 }
 
@@ -1617,9 +1649,9 @@ planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, con
     t2m_warm_cell<T2M_THREADS>(lmap, h, ystride, polstride, threadIdx.x);
     __syncthreads();
 
-    // The waves take the chunk's entries in batches of 64 from a block counter (one per round), so
-    // that they finish together: entries differ in cost (owned groups, window forms), and an even
-    // static split left the block waiting on its slowest wave (measured: 3.4% of the kernel).
+    // The waves take the chunk's entries in batches (claim_batch()) from a block counter (one per
+    // round), so that they finish together: entries differ in cost (owned groups, window forms),
+    // and an even static split left the block waiting on its slowest wave (3.4% of the kernel).
     float inv_scale;
     int round = 0;
     const bool finite = t2m_accumulate<T2M_THREADS>(
@@ -1627,16 +1659,16 @@ planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, con
         [&](bool fin, float scale, float limit, float &mrun, float &tsum, uint32_t &nwide, Checks &chk, bool first) {
             uint32_t *claim = s_words + W_CLAIM + min(round, 2);
             round++;
-            const uint32_t n = ch.end - ch.begin;
+            const uint32_t n = ch.end - ch.begin, bs = claim_batch(n, T2M_THREADS / WAVE);
             for (;;) {
                 uint32_t b = 0;
                 if (lane == 0)
-                    b = atomicAdd(claim, 64u);
+                    b = atomicAdd(claim, bs);
                 b = __builtin_amdgcn_readfirstlane(b);
                 if (b >= n)
                     break;
                 t2m_entries<Periodic, Vec, HasResp, TT, Debug>(acc, facc, tod, xp, plan_mt, qmask, nsamp, ch.begin + b,
-                                                               ch.begin + min(b + 64u, n), sct, g, ybase, xbase, gn,
+                                                               ch.begin + min(b + bs, n), sct, g, ybase, xbase, gn,
                                                                fin, scale, limit, mrun, tsum, nwide, chk, err, dbg,
                                                                first);
             }
@@ -1825,7 +1857,8 @@ batched_tod2map_kernel(float *__restrict__ lmap, const BatchTod *__restrict__ to
         t2m_warm_cell<T2M_THREADS>(lmap, h, ba.ystride, ba.polstride, opaque_tid());
         __syncthreads();
 
-        // Entries in batches of 64 from a block counter per round, as in the plain kernel.
+        // Entries in batches from a block counter per round, as in the plain kernel.
+        const uint32_t bs = claim_batch(it.nent, T2M_THREADS / WAVE);
         float inv_scale;
         int round = 0;
         const bool finite = t2m_accumulate<T2M_THREADS>(
@@ -1836,11 +1869,11 @@ batched_tod2map_kernel(float *__restrict__ lmap, const BatchTod *__restrict__ to
                 for (;;) {
                     uint32_t b0 = 0;
                     if (lane == 0)
-                        b0 = atomicAdd(claim, 64u);
+                        b0 = atomicAdd(claim, bs);
                     b0 = __builtin_amdgcn_readfirstlane(b0);
                     if (b0 >= it.nent)
                         break;
-                    batch_wave_segments(segs, it, b0, min(b0 + 64u, it.nent),
+                    batch_wave_segments(segs, it, b0, min(b0 + bs, it.nent),
                                         [&](const BatchSeg &sg, uint32_t b, uint32_t e) {
                         const BatchTod td = tods[sg.tod];
                         t2m_entries<Periodic, true, HasResp, TT>(acc, facc, static_cast<const TT *>(td.tod), td.xp,
