@@ -68,6 +68,7 @@ __device__ inline void dbg_tile(const Dbg &d, int o, int extra = HS + 1)
 
 constexpr int M2T_THREADS = 512;
 constexpr int T2M_THREADS = 1024;
+constexpr int T2M_LIST = 64 * 8;     // per wave: the (entry, quad) items of a 64-entry batch
 
 constexpr uint32_t FINITE_LIMIT = 0x7f800000u;   // bits of +Inf
 
@@ -1161,6 +1162,20 @@ __device__ inline float t2m_quad_fixed(unsigned long long *__restrict__ acc, con
     return probe;
 }
 
+// Exclusive prefix sum over the wave of n (0 <= n <= 15) and its total, from four ballots of the
+// bits of n (no LDS traffic).
+__device__ inline void wave_compact_offsets(uint32_t n, uint32_t &excl, uint32_t &total)
+{
+    excl = 0;
+    total = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        const uint64_t m = __ballot((n >> b) & 1u);
+        excl += uint32_t(__builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u))) << b;
+        total += uint32_t(__popcll(m)) << b;
+    }
+}
+
 // Largest |t| (times the gains) over the first 8 plan entries of [wb, we), for the calling
 // lane (lane 8*li + q reads quad q of entry wb + li); 'act' tells whether the lane read samples.
 template<bool Vec, bool HasResp, typename TT>
@@ -1236,34 +1251,48 @@ __device__ inline void t2m_entries(unsigned long long *__restrict__ acc, float *
                                    int64_t nsamp, uint32_t wb, uint32_t we, const float2 *__restrict__ sct,
                                    const Geom &g, int ybase, int xbase, const Gains &gn, bool finite, float scale,
                                    float limit, float &mrun, float &tsum, uint32_t &nwide, Checks &chk, uint32_t &err,
-                                   const Dbg &dbg = Dbg{}, bool count = false)
+                                   uint16_t *__restrict__ wlist, const Dbg &dbg = Dbg{}, bool count = false)
 {
     constexpr bool Q16 = !std::is_same<TT, float>::value;
     using IO = TodIO<TT, Vec>;
     const int lane = threadIdx.x & 63;
-    const int li = lane >> 3, q = lane & 7;
     const uint32_t ncl = uint32_t((nsamp + 31) >> 5);     // the last line may be partial
 
     for (uint32_t base = wb; base < we; base += 64) {
         const uint32_t cnt = min(64u, we - base);
-        const uint32_t packed = (uint32_t(lane) < cnt) ? pack_entry(plan_mt[base + lane]) : 0u;
-        const uint32_t qmw = (uint32_t(lane) < cnt) ? uint32_t(qmask[base + lane]) : 0u;
+        const bool valid = uint32_t(lane) < cnt;
+        const uint32_t packed = valid ? pack_entry(plan_mt[base + lane]) : 0u;
+        const uint32_t qmw = valid ? uint32_t(qmask[base + lane]) : 0u;
+        const uint32_t icl_l = packed & ICL_MASK;
+        if (valid && icl_l >= ncl)
+            err |= ERR_INCONSISTENT_NMT;
+        if constexpr (Debug) {
+            if (count && valid)
+                atomicAdd(dbg.entry_visits + (base + lane), 1u);
+        }
+        // The batch's evaluated 4-sample groups (the quad bits of the entries' masks), compacted
+        // into the wave's LDS list as items (entry, quad) in entry order, so that every lane of a
+        // step has a group: with one lane per (entry, quad) of 8 entries, the quads that an entry
+        // does not evaluate left lanes idle (13% of them on a raster, 20% on fast rotated scans).
+        const uint32_t qb = (valid && icl_l < ncl) ? (qmw & 0xFFu) : 0u;
+        uint32_t pos, total;
+        wave_compact_offsets(__popc(qb), pos, total);
+        for (uint32_t bits = qb; bits; bits &= bits - 1)
+            wlist[pos++] = uint16_t((uint32_t(lane) << 3) | uint32_t(__builtin_ctz(bits)));
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
-        for (uint32_t j = 0; j < cnt; j += 8) {
-            const uint32_t idx = j + li;
-            const uint32_t e2 = __shfl(packed, int(min(idx, 63u)), WAVE);
-            const uint32_t qm = __shfl(qmw, int(min(idx, 63u)), WAVE);
+        for (uint32_t r = 0; r < total; r += 64) {
+            const uint32_t item = r + uint32_t(lane);
+            const uint32_t code = (item < total) ? uint32_t(wlist[item]) : 0u;
+            const int li = int(code >> 3), q = int(code & 7u);
+            const uint32_t e2 = __shfl(packed, li, WAVE);
+            const uint32_t qm = __shfl(qmw, li, WAVE);
             // Columns of the right-hand cell that this entry evaluates (ownership rule at HALO_X).
             const unsigned lxlim = (qm & QM_TAKES_RIGHT) ? unsigned(64 + HALO_X - 1) : 64u;
             const uint32_t icl = e2 & ICL_MASK;
-            const bool act = (idx < cnt) && (icl < ncl);
-            if (idx < cnt && icl >= ncl)
-                err |= ERR_INCONSISTENT_NMT;
-            if constexpr (Debug) {
-                if (count && q == 0 && idx < cnt)
-                    atomicAdd(dbg.entry_visits + (base + idx), 1u);
-            }
-            if (!act || !((qm >> q) & 1u))
+            if (item >= total)
                 continue;
             const int64_t s0 = int64_t(icl) * LINE + 4 * q;
             // Past the end of the TOD (nv < 4) the samples read as zeros: they deposit nothing.
@@ -1340,6 +1369,8 @@ __device__ inline void t2m_entries(unsigned long long *__restrict__ acc, float *
                 }
             }
         }
+        // Every lane has read its items before the next batch rewrites the list.
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -1616,6 +1647,7 @@ planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, con
     __shared__ __align__(16) unsigned long long acc[3 * HPLANE];       // 116 KiB
     __shared__ float2 sct[SCT];
     __shared__ uint32_t s_words[T2M_WORDS];
+    __shared__ uint16_t s_list[T2M_THREADS / WAVE][T2M_LIST];            // 16 KiB
     float *facc = reinterpret_cast<float *>(acc);         // float tile for the non-finite fallback
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1669,8 +1701,8 @@ planned_tod2map_kernel(float *__restrict__ lmap, const TT *__restrict__ tod, con
                     break;
                 t2m_entries<Periodic, Vec, HasResp, TT, Debug>(acc, facc, tod, xp, plan_mt, qmask, nsamp, ch.begin + b,
                                                                ch.begin + min(b + bs, n), sct, g, ybase, xbase, gn,
-                                                               fin, scale, limit, mrun, tsum, nwide, chk, err, dbg,
-                                                               first);
+                                                               fin, scale, limit, mrun, tsum, nwide, chk, err,
+                                                               s_list[wave], dbg, first);
             }
         },
         inv_scale, err, threadIdx.x);
@@ -1813,6 +1845,7 @@ batched_tod2map_kernel(float *__restrict__ lmap, const BatchTod *__restrict__ to
     __shared__ __align__(16) unsigned long long acc[3 * HPLANE];       // 116 KiB
     __shared__ float2 sct[SCT];
     __shared__ uint32_t s_words[T2M_WORDS], s_item;
+    __shared__ uint16_t s_list[T2M_THREADS / WAVE][T2M_LIST];            // 16 KiB
     float *facc = reinterpret_cast<float *>(acc);
 
     fill_sincos_table(sct, T2M_THREADS);
@@ -1879,7 +1912,7 @@ batched_tod2map_kernel(float *__restrict__ lmap, const BatchTod *__restrict__ to
                         t2m_entries<Periodic, true, HasResp, TT>(acc, facc, static_cast<const TT *>(td.tod), td.xp,
                                                                td.plan_mt, td.qmask, td.nsamp, b, e, sct, g, ybase,
                                                                xbase, batch_gains(td), fin, scale, limit, mrun, tsum,
-                                                               nwide, chk, err);
+                                                               nwide, chk, err, s_list[wave]);
                     });
                 }
             },
