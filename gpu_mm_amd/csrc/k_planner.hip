@@ -92,6 +92,22 @@ __device__ inline bool extract_one(uint32_t k[4], uint32_t &c, bool &lo_hit, boo
     return true;
 }
 
+// Fast path of the extraction (the usual case, a line inside one cell): true (wave-uniform) when
+// in both halves every sample's footprint is the one cell of the half's first sample, k[0] of
+// lane 0 or 32. Then that cell, in 'c', is the half's only cell, the one extract_one() would
+// return first, and nlo / nhi count the halves whose line exists (first sample active).
+__device__ inline bool single_cell_halves(const uint32_t k[4], uint32_t &c, uint32_t &nlo, uint32_t &nhi)
+{
+    const uint32_t lo = __builtin_amdgcn_readlane(k[0], 0), hi = __builtin_amdgcn_readlane(k[0], 32);
+    c = (lane_id() < 32) ? lo : hi;
+    nlo = (lo != NO_CELL);
+    nhi = (hi != NO_CELL);
+    // Inactive lanes (past the end of a partial line, or of a line that does not exist) hold
+    // NO_CELL in every slot.
+    const bool one = ((k[1] & k[2] & k[3]) == NO_CELL) && (k[0] == c || k[0] == NO_CELL);
+    return __ballot(!one) == 0;
+}
+
 template<bool Periodic>
 __device__ inline uint32_t sample_cells(float y, float x, int ny, int nx, bool active, uint32_t k[4])
 {
@@ -141,7 +157,11 @@ preplan_count_kernel(const float *__restrict__ xp, int64_t nsamp, int64_t ncl, i
         for (int u = 0; u < PP_UNROLL; u++) {
             uint32_t k[4];
             err |= sample_cells<Periodic>(y[u], x[u], ny, nx, act[u], k);
-            uint32_t c;
+            uint32_t c, nlo, nhi;
+            if (single_cell_halves(k, c, nlo, nhi)) {
+                cnt += nlo + nhi;
+                continue;
+            }
             bool lo, hi;
             while (extract_one(k, c, lo, hi))
                 cnt += uint32_t(lo) + uint32_t(hi);
@@ -206,8 +226,10 @@ constexpr int EM_NW = EM_THREADS / WAVE;
 constexpr int EM_U = 2;                       // line pairs per wave per step
 constexpr int EM_LINES = 2 * EM_NW * EM_U;    // lines per block step (32)
 
+// At most 64 VGPRs (8 waves per SIMD, 4 blocks per CU): the planner grid (1024 blocks at most)
+// is then resident at once on 256 CUs instead of leaving a tail.
 template<bool Periodic>
-__global__ void __launch_bounds__(EM_THREADS)
+__global__ void __launch_bounds__(EM_THREADS) __attribute__((amdgpu_waves_per_eu(8)))
 plan_emit_kernel(const float *__restrict__ xp, int64_t nsamp, int64_t ncl, int64_t ncl_per_block,
                  int ny, int nx, const uint32_t *__restrict__ nmt_cumsum,
                  uint64_t *__restrict__ out, uint32_t *__restrict__ err_out)
@@ -246,14 +268,20 @@ plan_emit_kernel(const float *__restrict__ xp, int64_t nsamp, int64_t ncl, int64
         for (int u = 0; u < EM_U; u++) {
             uint32_t k[4];
             err |= sample_cells<Periodic>(y[u], x[u], ny, nx, act[u], k);
-            uint32_t c, nlo = 0, nhi = 0;
-            bool lo, hi;
+            uint32_t c, nlo, nhi;
             mycell[u] = NO_CELL;
-            while (extract_one(k, c, lo, hi)) {
-                uint32_t j = half ? nhi : nlo;
-                mycell[u] = (uint32_t(sub) == j) ? c : mycell[u];
-                nlo += lo;
-                nhi += hi;
+            if (single_cell_halves(k, c, nlo, nhi)) {
+                mycell[u] = (sub == 0) ? c : NO_CELL;
+            }
+            else {
+                nlo = nhi = 0;
+                bool lo, hi;
+                while (extract_one(k, c, lo, hi)) {
+                    uint32_t j = half ? nhi : nlo;
+                    mycell[u] = (uint32_t(sub) == j) ? c : mycell[u];
+                    nlo += lo;
+                    nhi += hi;
+                }
             }
             nmy[u] = half ? nhi : nlo;
             if (sub == 0)
