@@ -898,24 +898,6 @@ struct QuadSamples
     float ts[4], ps[4], av[4];
 };
 
-// Row and column range of the evaluated samples among K0..K1-1 (ey < 0 when there are none).
-template<int K0, int K1>
-__device__ inline void quad_extent(const QuadSamples &q, int &by, int &bx, int &ey, int &ex)
-{
-    by = 64;
-    bx = 1 << 20;
-    ey = ex = -1;
-#pragma unroll
-    for (int k = K0; k < K1; k++) {
-        if (q.own[k]) {
-            by = min(by, q.ly[k]);
-            ey = max(ey, q.ly[k]);
-            bx = min(bx, q.lx[k]);
-            ex = max(ex, q.lx[k]);
-        }
-    }
-}
-
 // Deposits of samples K0..K1-1 whose corners span at most two rows and two columns (rows by..ey,
 // columns bx..ex): their bilinear footprint lies in a 3x3 pixel window, summed per pixel in
 // registers and added with one LDS atomic per touched pixel and plane (18 when the samples stay
@@ -1071,10 +1053,11 @@ __device__ inline float t2m_window(unsigned long long *__restrict__ acc, const f
 
 // Fixed-point tod2map of one 4-sample group (t already scaled): one 3x3 window (t2m_window())
 // when, in every lane of the wave, the evaluated samples' corners span at most two rows and two
-// columns (scan speeds below ~0.33 pixel per sample); otherwise a window per pair of samples
-// (samples 0-1 and 2-3, which fit one below 1 pixel per sample), and sample by sample for a
-// lane whose pair does not fit. The choice is per wave, so a wave runs one of the forms. Returns
-// a value that is NaN iff a value the block deposits for the group is NaN (an evaluated sample
+// columns (scan speeds below ~0.33 pixel per sample); otherwise sample by sample (four corners
+// each). The choice is per wave, so a wave runs one of the forms. (A window per sample pair for
+// the faster scans costs more: its 3x3 sums take 27 FMAs per sample when some lane of the wave
+// spans three rows or columns, against 12 products, for about as many LDS atomics.) Returns a
+// value that is NaN iff a value the block deposits for the group is NaN (an evaluated sample
 // with t, or t times a gain, NaN): the caller's check for the float fallback.
 template<bool Periodic, bool HasResp, bool Small, bool Debug = false>
 __device__ inline float t2m_quad_fixed(unsigned long long *__restrict__ acc, const float2 *__restrict__ tab,
@@ -1130,18 +1113,7 @@ __device__ inline float t2m_quad_fixed(unsigned long long *__restrict__ acc, con
     const bool tiny = near && !__any(!(dmax <= 1.0e-4f));
     if (!__any(!(ey - by <= 1 && ex - bx <= 1)))
         return t2m_window<HasResp, Small, Debug, 0, 4>(acc, tab, q, by, bx, ey, ex, s0, c0, x0, near, tiny, dbg);
-    int by1, bx1, ey1, ex1;
-    quad_extent<0, 2>(q, by, bx, ey, ex);
-    quad_extent<2, 4>(q, by1, bx1, ey1, ex1);
-    const bool fit0 = ey - by <= 1 && ex - bx <= 1, fit1 = ey1 - by1 <= 1 && ex1 - bx1 <= 1;
     float probe = 0.0f;
-    if (fit0 && fit1) {
-        if (ey >= 0)
-            probe += t2m_window<HasResp, Small, Debug, 0, 2>(acc, tab, q, by, bx, ey, ex, s0, c0, x0, near, tiny, dbg);
-        if (ey1 >= 0)
-            probe += t2m_window<HasResp, Small, Debug, 2, 4>(acc, tab, q, by1, bx1, ey1, ex1, s0, c0, x0, near, tiny, dbg);
-        return probe;
-    }
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         if (!q.own[k])
