@@ -52,9 +52,9 @@ def make_instance(kind, seed):
         return xp, ny, nx, True
     if kind == "mixed":
         # Detectors at different scan speeds and angle rates, so that the waves of one cell mix
-        # lanes whose 4-sample groups fit one 3x3 tod2map window (slow), fit one window per pair
-        # of samples (0.35-0.9 px/sample) or neither (above 1 px/sample), with angle steps of the
-        # first-order, second-order and table regimes.
+        # lanes whose 4-sample groups fit one 3x3 tod2map window (below ~0.33 px/sample) with
+        # lanes whose groups do not (deposited sample by sample, up to 2.5 px/sample), with angle
+        # steps of the first-order, second-order and table regimes.
         ndet, nt, ny, nx = 12, 32 * 256, 256, 512
         t = np.arange(nt, dtype=np.float64)
         speeds = [0.05, 0.2, 0.3, 0.4, 0.55, 0.7, 0.9, 1.3, 0.12, 0.6, 2.5, 0.33]
