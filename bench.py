@@ -150,25 +150,34 @@ def time_sync_errflags(torch, gmm, lmap, out_map, xp, plan, tod, out, steps, war
         gmm.set_errflag_check(False)
 
 
-def hbm_ceiling(torch, dev, nbytes=1 << 31, reps=10):
-    """Measured bandwidth ceilings on this GPU: a device copy (reads and writes nbytes) and a
-    read-only reduction (torch.sum over nbytes), median over reps, HIP events."""
+def hbm_ceiling(torch, gmm, dev, nbytes=1 << 32, reps=10):
+    """Measured bandwidth ceilings on this GPU, median over reps, HIP events: a device copy
+    (reads and writes nbytes), the library's streaming-read probe (reads nbytes with the
+    non-temporal 16-byte loads of the operators' input streams) and, for comparison, a
+    read-only reduction (torch.sum over nbytes)."""
     src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
     dst = torch.empty_like(src)
+    acc = torch.zeros(1, dtype=torch.float32, device=dev)
     out = {}
     for name, fn, moved in (("copy", lambda: dst.copy_(src), 2 * nbytes),
-                            ("read", lambda: torch.sum(src), nbytes)):
+                            ("read", lambda: gmm.stream_read(src, acc), nbytes),
+                            ("torch_sum", lambda: torch.sum(src), nbytes)):
         fn()
         ts = []
         for _ in range(reps):
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
+            # One untimed call first keeps the GPU busy while the start event and the timed
+            # calls are queued: otherwise the host's launch latency after the start event
+            # (tens of microseconds) counts against a 0.3 ms kernel.
             fn()
+            e0.record()
+            for _ in range(4):
+                fn()
             e1.record()
             e1.synchronize()
-            ts.append(e0.elapsed_time(e1) * 1e-3)
+            ts.append(e0.elapsed_time(e1) * 1e-3 / 4)
         out[name + "_gb_s"] = round(moved / float(np.median(ts)) / 1e9, 1)
-    del src, dst
+    del src, dst, acc
     torch.cuda.empty_cache()
     return out
 
@@ -616,8 +625,9 @@ def main():
                       "sync_gsamp_s": round(nsamp / (t_sync / args.steps) / 1e9, 3)}
     log(f"[bench] f32 step with synchronous error-flag checks: {errflag_timing['sync_ms_per_step']:.3f} ms "
         f"({errflag_timing['sync_gsamp_s']:.1f} Gsamp/s); deferred: {errflag_timing['deferred_ms_per_step']:.3f} ms")
-    ceiling = hbm_ceiling(torch, dev)
-    log(f"[bench] measured ceilings: device copy {ceiling['copy_gb_s']:.0f} GB/s, read-only sum {ceiling['read_gb_s']:.0f} GB/s")
+    ceiling = hbm_ceiling(torch, gmm, dev)
+    log(f"[bench] measured ceilings: device copy {ceiling['copy_gb_s']:.0f} GB/s, streaming read "
+        f"{ceiling['read_gb_s']:.0f} GB/s (torch.sum {ceiling['torch_sum_gb_s']:.0f} GB/s)")
 
     errs = form_errors(torch, gmm, w, forms, min(args.err_ndet, args.ndet)) if args.err_ndet > 0 else {}
     ok = [f for f in forms if f == "f32" or (f in errs and errs[f]["map_rms_rel"] <= MAP_ERR_BAR)]

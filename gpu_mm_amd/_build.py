@@ -42,6 +42,7 @@ LIB_SOURCES = [
     ("k_dynmap.hip", []),
     ("k_cells.hip", []),
     ("k_debug.hip", []),
+    ("k_probe.hip", []),
 ]
 
 LIB_NAME = "libgpu_mm_amd.so"

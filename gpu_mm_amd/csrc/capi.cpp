@@ -316,6 +316,22 @@ int gmm_kernel_timer_read(int which, double *total_ms, int64_t *count)
     });
 }
 
+int gmm_stream_read(const float *src_gpu, int64_t n, float *sum_gpu, void *stream_)
+{
+    return guarded([&] {
+        const char *where = "stream_read";
+        require_ptr(src_gpu, where, "src");
+        require_ptr(sum_gpu, where, "sum");
+        if (n < 0 || n % 4 != 0)
+            fail(std::string(where) + ": n must be a non-negative multiple of 4");
+        if (reinterpret_cast<uintptr_t>(src_gpu) & 15)
+            fail(std::string(where) + ": 'src' must be 16-byte aligned");
+        if (n == 0)
+            return;
+        launch_stream_read(src_gpu, n, sum_gpu, as_stream(stream_));
+    });
+}
+
 // ------------------------------------------------------------------------------------------
 
 int gmm_lpix_create(gmm_lpix **out, int64_t nypix_global, int64_t nxpix_global,

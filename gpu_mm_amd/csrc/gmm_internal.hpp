@@ -272,6 +272,10 @@ void launch_cell_reduce(float *dst, int64_t ndst, const float *src, int64_t nsrc
                         uint32_t *bad, hipStream_t stream);
 void launch_local_to_global(float *dst, const float *src, const gmm_lpix &lp, hipStream_t stream);
 
+// Streaming-read probe (k_probe.hip, a measurement aid): reads n floats (n % 4 == 0, 16-byte
+// aligned) with non-temporal 16-byte loads and adds their sum to *sum.
+void launch_stream_read(const float *p, int64_t n, float *sum, hipStream_t stream);
+
 // Per-row 16-bit encoding of a float32 TOD [nrows][row_len]: scale[r] = max(max_s |tod[r][s]| /
 // (0.5 * largest finite value of the type), FLT_MIN), out[r][s] = tod[r][s] / scale[r] rounded
 // to nearest even, with error feedback of order 0 (none), 1 or 2 along each row.

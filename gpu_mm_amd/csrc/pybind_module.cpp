@@ -782,6 +782,17 @@ PYBIND11_MODULE(_gpu_mm_amd_py, m)
         check(gmm_kernel_timer_read(which, &ms, &n));
         return py::make_tuple(ms, n);
     }, "Returns (total_ms, launches) of GMM_TIMED_MAP2TOD (0) or GMM_TIMED_TOD2MAP (1) since the last enable/read");
+    m.def("stream_read", [](py::object src, py::object sum, uintptr_t stream) {
+        const char *where = "stream_read";
+        Arr a = get_arr(src, where, "src"), b = get_arr(sum, where, "sum");
+        require(a, where, "src", kDLFloat, 32, true, "float32");
+        require(b, where, "sum", kDLFloat, 32, true, "float32");
+        if (b.size < 1)
+            raise(std::string(where) + ": 'sum' must hold at least one float");
+        py::gil_scoped_release nogil;
+        check(gmm_stream_read(static_cast<const float *>(a.ptr), a.size, static_cast<float *>(b.ptr), as_stream(stream)));
+    }, py::arg("src"), py::arg("sum"), py::arg("stream") = 0,
+       "Reads src once with non-temporal 16-byte loads and adds its sum to sum[0] (bandwidth probe)");
     py::class_<gmm_batch_config>(m, "BatchConfig")
         .def(py::init([]() { gmm_batch_config c; gmm_batch_default_config(&c); return c; }))
         .def_readwrite("m2t_item_entries", &gmm_batch_config::m2t_item_entries)

@@ -131,6 +131,14 @@ def kernel_timer_read(kernel):
     return _ext.kernel_timer_read({"map2tod": 0, "tod2map": 1}[kernel])
 
 
+def stream_read(src, out):
+    """Bandwidth probe (a measurement aid): reads the float32 GPU tensor 'src' (contiguous,
+    numel % 4 == 0, 16-byte aligned) once with non-temporal 16-byte loads, the access of the
+    planned operators' input streams, and adds its sum to out[0]. Asynchronous on torch's
+    current stream."""
+    _ext.stream_read(src, out, stream=_stream())
+
+
 def map2tod(tod, local_map, xpointing, plan, partial_pixelization=False, response=None, debug=False, accum=False,
             tod_scale=None):
     """

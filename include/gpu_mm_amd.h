@@ -250,6 +250,14 @@ enum { GMM_TIMED_MAP2TOD = 0, GMM_TIMED_TOD2MAP = 1 };
 int gmm_kernel_timer_enable(int enable);
 int gmm_kernel_timer_read(int which, double *total_ms, int64_t *count);
 
+/* ---- Streaming-read probe -----------------------------------------------------------------
+ * No counterpart in the reference; a measurement aid for the benchmark's bandwidth ceilings.
+ * Reads src_gpu[0..n) once with non-temporal 16-byte loads (the access of the planned operators'
+ * pointing and TOD streams) and adds the sum of the values to *sum_gpu (a device float; atomic,
+ * so the order of the additions varies). n % 4 == 0, src_gpu 16-byte aligned. Asynchronous on
+ * 'stream'. */
+int gmm_stream_read(const float *src_gpu, int64_t n, float *sum_gpu, void *stream);
+
 /* ---- 16-bit TODs -------------------------------------------------------------------------
  * No counterpart in the reference (which stores TODs as float32 only). A 16-bit TOD of nsamp
  * samples in tod_nrows rows holds value / tod_scale[row] as FP16 or BF16, with one FP32 scale

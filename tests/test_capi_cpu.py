@@ -32,6 +32,28 @@ def test_library_exports_every_declared_symbol():
     assert lib.gmm_version() >= 100
 
 
+def test_stream_read_argument_checks():
+    """The bandwidth probe's argument checks run before any GPU call: a length that is not a
+    multiple of 4, a source that is not 16-byte aligned and null pointers return an error with a
+    message; n = 0 launches nothing."""
+    import torch  # noqa: F401
+    from gpu_mm_amd import _native
+    lib = ctypes.CDLL(_native.lib_path())
+    lib.gmm_stream_read.restype = ctypes.c_int
+    lib.gmm_stream_read.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
+    lib.gmm_last_error.restype = ctypes.c_char_p
+    # Host addresses stand in for device pointers: none of these calls reaches the GPU.
+    buf = (ctypes.c_float * 16)()
+    base = (ctypes.addressof(buf) + 15) & ~15
+    acc = ctypes.addressof(buf)
+    for src, n, sum_, text in ((base, 6, acc, b"multiple of 4"), (base, -4, acc, b"multiple of 4"),
+                               (base + 4, 8, acc, b"16-byte aligned"), (None, 8, acc, b"src"),
+                               (base, 8, None, b"sum")):
+        assert lib.gmm_stream_read(src, n, sum_, None) != 0
+        assert text in lib.gmm_last_error()
+    assert lib.gmm_stream_read(base, 0, acc, None) == 0
+
+
 def test_extension_imports(gmm):
     assert gmm.PointingPrePlan.preplan_size == 1024
     assert gmm.gpu_mm._ext._get_tsize() == 4

@@ -1022,3 +1022,23 @@ def test_debug_checks_detect_corrupted_plans(gmm, dev):
         plan.check_invariants()
     # The untouched plan passes.
     gmm.PointingPlan(pre, xg).check_invariants()
+
+
+# ------------------------------------------------------------------------------------------
+# The streaming-read probe behind the benchmark's bandwidth ceiling.
+def test_stream_read_probe(gmm, dev):
+    """Every element is read once: with 0/1 values all partial sums are exact in float32, so
+    the probe's sum equals the count of ones whatever the order of its atomic additions. Lengths
+    below, at and above one grid's worth; misaligned or ragged inputs raise."""
+    g = torch.Generator(device="cpu").manual_seed(7)
+    for n in (4, 4 * 1001, 1 << 22, (1 << 22) + 4 * 37):
+        print(f"\n[stream_read] n={n}", flush=True)
+        src = torch.randint(0, 2, (n,), generator=g).to(torch.float32).to(dev)
+        acc = torch.zeros(1, dtype=torch.float32, device=dev)
+        gmm.stream_read(src, acc)
+        gmm.stream_read(src, acc)          # adds: two reads give twice the count
+        assert acc.item() == 2 * int(src.sum().item())
+    with pytest.raises(RuntimeError, match="16-byte aligned"):
+        gmm.stream_read(src[1:9], acc)
+    with pytest.raises(RuntimeError, match="multiple of 4"):
+        gmm.stream_read(src[:6], acc)
