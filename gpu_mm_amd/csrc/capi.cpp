@@ -475,7 +475,7 @@ int gmm_preplan_create(gmm_preplan **out, const float *xpointing_gpu, int64_t ns
         pp.pointing_nblocks = (nmt + pp.nmt_per_threadblock - 1) / pp.nmt_per_threadblock;
         pp.overhead = 32.0 * double(nmt) / double(nsamp) - 1.0;
 
-        // This build's plan layout.
+        // The plan layout of this library.
         pp.nycells_g = (nypix_global + 63) / 64;
         pp.nxcells_g = (nxpix_global + 63) / 64;
         const int64_t nkeys = pp.nycells_g * pp.nxcells_g;
