@@ -305,7 +305,7 @@ def py_plan(xp, ny, nx, periodic):
     return np.array(entries, dtype=np.uint64), lines, int(np.bitwise_or.reduce(err))
 
 
-# n = 32 * 700 - 13 and 32 * 700 + 4: a partial last line (nsamp % 32 != 0, which this build
+# n = 32 * 700 - 13 and 32 * 700 + 4: a partial last line (nsamp % 32 != 0, which the library
 # accepts and the reference does not).
 @pytest.mark.parametrize("periodic", [False, True])
 @pytest.mark.parametrize("n", [32 * 700, 32 * 700 - 13, 32 * 700 + 4])
