@@ -1,0 +1,85 @@
+"""Randomized GPU parity on small instances: random map sizes, scans back and forth at 0.01-3
+pixels per sample in any direction (drifting and wrapping around in periodic x), detector
+offsets, angles that are constant, slowly or quickly turning or random per sample, and TOD lengths
+that are not multiples of 32 or of 4. The planner (preplan counts, plan bit for bit) and the
+planned operators (FP32 and FP64 oracle bars, device-side invariant checks of the debug mode on)
+against the oracle, as in tests/test_gpu_ops.py."""
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+F32 = np.float32
+F64_BAR = 1e-5
+
+
+def l1rel(a, b):
+    a = np.asarray(a, dtype=np.float64).reshape(-1)
+    b = np.asarray(b, dtype=np.float64).reshape(-1)
+    return float(np.sum(np.abs(a - b)) / (np.sum(np.abs(a)) + np.sum(np.abs(b))))
+
+
+def make_random_instance(seed):
+    """(xpointing (3, ndet, nt) float32, ny, nx, periodic) from the seed."""
+    rng = np.random.default_rng(5000 + seed)
+    periodic = bool(rng.integers(2))
+    ny, nx = int(rng.integers(70, 400)), int(rng.integers(70, 500))
+    ndet, nt = int(rng.integers(1, 7)), int(rng.integers(100, 3000))
+    t = np.arange(nt, dtype=np.float64)
+    amp = 0.4 * min(ny, nx)
+    xp = np.empty((3, ndet, nt), F32)
+    for d in range(ndet):
+        v = float(np.exp(rng.uniform(np.log(0.01), np.log(3.0))))     # pixels per sample
+        phi = rng.uniform(0.0, 2.0 * np.pi)
+        u = (v * t + rng.uniform(0.0, 2.0 * amp)) % (2.0 * amp)
+        u = np.where(u < amp, u, 2.0 * amp - u) - 0.5 * amp            # back and forth, |u| <= amp/2
+        y = 0.5 * ny + u * np.sin(phi) + rng.uniform(-3.0, 3.0)
+        x = 0.5 * nx + u * np.cos(phi) + rng.uniform(-3.0, 3.0)
+        if periodic:
+            x = x + rng.uniform(-0.5, 0.5) * t                          # drift, wrapped below
+            x = np.mod(x + 0.5 * nx, 2.0 * nx) - 0.5 * nx
+        kind = int(rng.integers(4))
+        a0 = rng.uniform(0.0, np.pi)
+        if kind == 3:
+            a = rng.uniform(-10.0, 10.0, nt)
+        else:
+            a = a0 + (0.0, 1e-6, 2e-3)[kind] * t
+        xp[0, d], xp[1, d], xp[2, d] = y, x, a
+    return xp, ny, nx, periodic
+
+
+@pytest.mark.parametrize("seed", range(48))
+def test_random_instance_parity(gmm, orc, dev, seed):
+    xp, ny, nx, periodic = make_random_instance(seed)
+    ndet, nt = xp.shape[1], xp.shape[2]
+    print(f"\n[random instance seed={seed}] ndet={ndet} nt={nt} ny={ny} nx={nx} periodic={periodic}", flush=True)
+    xg = torch.from_numpy(xp).to(dev)
+    pre = gmm.PointingPrePlan(xg, ny, nx, periodic_xcoord=periodic)
+    plan = gmm.PointingPlan(pre, xg, debug=True)
+    cum, nmt, err = orc.plan_count(xp, ny, nx, periodic)
+    assert err == 0 and pre.plan_nmt == nmt
+    assert np.array_equal(pre.get_nmt_cumsum(), cum)
+    ref_mt, err = orc.plan_build(xp, ny, nx, periodic)
+    assert err == 0 and np.array_equal(plan.get_plan_mt(), ref_mt)
+
+    rng = np.random.default_rng(seed)
+    lp = gmm.LocalPixelization.make_rectangle(ny, nx, periodic_xcoord=periodic)
+    pix = orc.Pix.rectangle(ny, nx, periodic)
+    m = rng.standard_normal(3 * lp.npix).astype(F32)
+    mg = gmm.LocalMap(lp, torch.from_numpy(m).to(dev))
+    ref32, _ = orc.map2tod(m, xp, pix)
+    ref64, _ = orc.map2tod(m.astype(np.float64), xp, pix, f64=True)
+    tg = torch.full((ndet, nt), 3.0, dtype=torch.float32, device=dev)
+    gmm.map2tod(tg, mg, xg, plan, debug=True)
+    got = tg.cpu().numpy()
+    assert l1rel(got, ref32) < 1e-6 and l1rel(got, ref64) < F64_BAR
+
+    d = rng.standard_normal((ndet, nt)).astype(F32)
+    mref32, _ = orc.tod2map(d, xp, pix)
+    mref64, _ = orc.tod2map(d.astype(np.float64), xp, pix, f64=True)
+    out = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
+    gmm.tod2map(out, torch.from_numpy(d).to(dev), xg, plan, debug=True)
+    got = out.arr.cpu().numpy()
+    assert l1rel(got, mref32) < 1e-6 and l1rel(got, mref64) < F64_BAR
