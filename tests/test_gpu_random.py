@@ -21,12 +21,10 @@ def l1rel(a, b):
     return float(np.sum(np.abs(a - b)) / (np.sum(np.abs(a)) + np.sum(np.abs(b))))
 
 
-def make_random_instance(seed):
-    """(xpointing (3, ndet, nt) float32, ny, nx, periodic) from the seed."""
-    rng = np.random.default_rng(5000 + seed)
-    periodic = bool(rng.integers(2))
-    ny, nx = int(rng.integers(70, 400)), int(rng.integers(70, 500))
-    ndet, nt = int(rng.integers(1, 7)), int(rng.integers(100, 3000))
+def random_scan(rng, ndet, nt, ny, nx, periodic):
+    """xpointing (3, ndet, nt) float32: per detector a scan back and forth at 0.01-3 pixels per
+    sample in a random direction (plus a wrapped drift in periodic x), a small offset, and
+    angles that are constant, slowly or quickly turning, or random per sample."""
     t = np.arange(nt, dtype=np.float64)
     amp = 0.4 * min(ny, nx)
     xp = np.empty((3, ndet, nt), F32)
@@ -47,7 +45,26 @@ def make_random_instance(seed):
         else:
             a = a0 + (0.0, 1e-6, 2e-3)[kind] * t
         xp[0, d], xp[1, d], xp[2, d] = y, x, a
-    return xp, ny, nx, periodic
+    return xp
+
+
+def make_random_instance(seed):
+    """(xpointing (3, ndet, nt) float32, ny, nx, periodic) from the seed."""
+    rng = np.random.default_rng(5000 + seed)
+    periodic = bool(rng.integers(2))
+    ny, nx = int(rng.integers(70, 400)), int(rng.integers(70, 500))
+    ndet, nt = int(rng.integers(1, 7)), int(rng.integers(100, 3000))
+    return random_scan(rng, ndet, nt, ny, nx, periodic), ny, nx, periodic
+
+
+def make_random_batch(seed):
+    """(list of xpointings with nsamp % 4 == 0, ny, nx, periodic): 3-5 TODs on one map."""
+    rng = np.random.default_rng(9000 + seed)
+    periodic = bool(rng.integers(2))
+    ny, nx = int(rng.integers(70, 400)), int(rng.integers(70, 500))
+    xps = [random_scan(rng, int(rng.integers(1, 6)), 4 * int(rng.integers(25, 750)), ny, nx, periodic)
+           for _ in range(int(rng.integers(3, 6)))]
+    return xps, ny, nx, periodic
 
 
 @pytest.mark.parametrize("seed", range(48))
@@ -83,3 +100,50 @@ def test_random_instance_parity(gmm, orc, dev, seed):
     gmm.tod2map(out, torch.from_numpy(d).to(dev), xg, plan, debug=True)
     got = out.arr.cpu().numpy()
     assert l1rel(got, mref32) < 1e-6 and l1rel(got, mref64) < F64_BAR
+
+
+# Work-item shapes of the batched operators: the defaults, many small items, and a persistent
+# grid with cells split across items (as in tests/test_gpu_batch.py).
+BATCH_CONFIGS = [
+    dict(),
+    dict(m2t_item_entries=64, m2t_threads=256, m2t_blocks_per_cu=0, t2m_item_entries=64, t2m_blocks_per_cu=0),
+    dict(m2t_item_entries=200, m2t_threads=512, m2t_blocks_per_cu=1, t2m_item_entries=300, t2m_blocks_per_cu=2),
+]
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_batch_parity(gmm, orc, dev, seed):
+    """Batched operators over 3-5 random TODs of uneven shapes on one map: map2tod of each TOD
+    and tod2map summed over the TODs against the oracle, for each work-item shape."""
+    xps, ny, nx, periodic = make_random_batch(seed)
+    print(f"\n[random batch seed={seed}] shapes={[x.shape[1:] for x in xps]} ny={ny} nx={nx} "
+          f"periodic={periodic}", flush=True)
+    lp = gmm.LocalPixelization.make_rectangle(ny, nx, periodic_xcoord=periodic)
+    pix = orc.Pix.rectangle(ny, nx, periodic)
+    xgs = [torch.from_numpy(x).to(dev) for x in xps]
+    plans = [gmm.PointingPlan(gmm.PointingPrePlan(x, ny, nx, periodic_xcoord=periodic), x) for x in xgs]
+    rng = np.random.default_rng(seed)
+    m = rng.standard_normal(3 * lp.npix).astype(F32)
+    mg = gmm.LocalMap(lp, torch.from_numpy(m).to(dev))
+    ref_tods = [orc.map2tod(m, x, pix)[0] for x in xps]
+    tods = [rng.standard_normal(x.shape[1:]).astype(F32) for x in xps]
+    ref32 = np.zeros(3 * lp.npix, F32)
+    ref64 = np.zeros(3 * lp.npix, np.float64)
+    for x, d in zip(xps, tods):
+        ref32, _ = orc.tod2map(d, x, pix, lmap=ref32)
+        ref64, _ = orc.tod2map(d.astype(np.float64), x, pix, lmap=ref64, f64=True)
+    for cfg in BATCH_CONFIGS:
+        conf = gmm.BatchConfig()
+        for k, v in cfg.items():
+            setattr(conf, k, v)
+        outs = [torch.full(x.shape[1:], 3.0, dtype=torch.float32, device=dev) for x in xps]
+        b = gmm.PlanBatch(lp, plans, xgs, outs, config=conf)
+        b.map2tod(mg, debug=True)
+        for o, r in zip(outs, ref_tods):
+            assert l1rel(o.cpu().numpy(), r) < 1e-6
+        for o, d in zip(outs, tods):
+            o.copy_(torch.from_numpy(d))
+        out = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
+        b.tod2map(out, debug=True)
+        got = out.arr.cpu().numpy()
+        assert l1rel(got, ref32) < 1e-6 and l1rel(got, ref64) < F64_BAR, cfg
