@@ -687,7 +687,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (GPU-generated raster pointing, GRF map, N(0,1) TOD)",
         "config": {
-            "workload": "planned map2tod+tod2map, single TOD 2048x262144 on sparse 3x4096x8192 map",
+            "workload": f"planned map2tod+tod2map, single TOD {args.ndet}x{args.nt} on sparse 3x{w['ny']}x{w['nx']} map",
             "ndet": args.ndet, "nt": args.nt, "nsamp": nsamp, "cells": ncells, "plan_nmt": int(plan.plan_nmt),
             "chunks": int(plan.nchunks), "chosen_form": chosen, "tod_storage": chosen,
             "tod16_error_feedback": TOD16_ERROR_FEEDBACK,
