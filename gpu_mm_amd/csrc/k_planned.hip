@@ -37,6 +37,7 @@
 // BF16) stored as value/scale[row] with one FP32 scale per TOD row; arithmetic stays FP32.
 
 #include "gmm_internal.hpp"
+#include "gmm_locate.hpp"
 
 namespace gmm {
 
@@ -340,80 +341,6 @@ __device__ inline void quad_scale(const Gains &g, uint32_t s0, float (&sc)[4])
 //   - periodic x ([-nx, 2nx]): the largest max(lo - x, x - hi) (> 0 iff out of range; these
 //     exact-sign subtractions have exact signs) and a sum of the coordinates (NaN iff some x is
 //     NaN).
-
-struct Geom
-{
-    int ny, nx;
-    float ymax;     // ny - 1
-    float xlo, xhi; // valid x range of the periodic case: [-nx, 2nx]
-    float fnx;
-    float fny2, fnx2; // ny - 2, nx - 2
-    uint32_t uymax, uxmax;  // bit patterns of ny - 1 and nx - 2
-};
-
-__device__ inline Geom make_geom(int ny, int nx, bool periodic)
-{
-    Geom g;
-    g.ny = ny;
-    g.nx = nx;
-    g.ymax = float(ny - 1);
-    g.fnx = float(nx);
-    g.fny2 = float(ny - 2);
-    g.fnx2 = float(nx - 2);
-    g.xlo = periodic ? -float(nx) : 0.0f;
-    g.xhi = periodic ? float(2 * nx) : float(nx - 2);
-    g.uymax = __float_as_uint(g.ymax);
-    g.uxmax = __float_as_uint(g.fnx2);
-    return g;
-}
-
-struct Checks
-{
-    uint32_t uy = 0, ux = 0;
-    float px = -INFINITY, pn = 0.0f;
-
-    __device__ uint32_t bits(const Geom &g) const
-    {
-        uint32_t e = 0;
-        if (uy > g.uymax)
-            e |= ERR_BAD_YPIX;
-        if (ux > g.uxmax || !(px <= 0.0f) || !(pn == pn))
-            e |= ERR_BAD_XPIX;
-        return e;
-    }
-};
-
-// Clamp to [0, hi] in one instruction (v_med3_f32). For a NaN input the hardware returns the
-// minimum of the operands, which is 0 here or NaN; either converts to index 0.
-__device__ inline float clamp0(float v, float hi)
-{
-    return __builtin_amdgcn_fmed3f(v, 0.0f, hi);
-}
-
-template<bool Periodic>
-__device__ inline void locate_op(float y, float x, const Geom &g, Checks &c, int &iy0, int &ix0, float &dy, float &dx)
-{
-    c.uy = max(c.uy, __float_as_uint(y + 0.0f));
-    const int iy = int(clamp0(y, g.fny2));
-    dy = y - float(iy);
-    iy0 = iy;
-
-    if (Periodic) {
-        c.px = fmaxf(c.px, fmaxf(g.xlo - x, x - g.xhi));
-        c.pn += x;
-        x = (x >= 0.0f) ? x : (x + g.fnx);
-        x = (x <= g.fnx) ? x : (x - g.fnx);
-        const int ix = int(clamp0(x, g.fnx));
-        dx = __builtin_amdgcn_fractf(x);                  // x - ix for x in [0, nx]
-        ix0 = (ix < g.nx) ? ix : 0;
-    }
-    else {
-        c.ux = max(c.ux, __float_as_uint(x + 0.0f));
-        const int ix = int(clamp0(x, g.fnx2));
-        dx = __builtin_amdgcn_fractf(x);                  // x - ix for x in [0, nx - 2]
-        ix0 = ix;
-    }
-}
 
 // ------------------------------------------------------------------------------------------
 // Halo tile (HROWS x HS per plane): the block's cell (64x64) plus row 64 from the cells below and

@@ -6,6 +6,7 @@
 // ballot/readlane extraction instead of warp32 match-any.
 
 #include "gmm_internal.hpp"
+#include "gmm_locate.hpp"
 
 namespace gmm {
 
@@ -108,17 +109,20 @@ __device__ inline bool single_cell_halves(const uint32_t k[4], uint32_t &c, uint
     return __ballot(!one) == 0;
 }
 
+// Footprint cells of one sample (locate_pixel's corners, include/gpu_mm_internals.hpp:124-164,
+// located by locate_op with its range checks deferred to 'chk'). An inactive lane holds NO_CELL
+// everywhere; its y = x = 0 lie in range and leave the checks unchanged.
 template<bool Periodic>
-__device__ inline uint32_t sample_cells(float y, float x, int ny, int nx, bool active, uint32_t k[4])
+__device__ inline void sample_cells(float y, float x, const Geom &g, Checks &chk, bool active, uint32_t k[4])
 {
     Pixel<float> p;
-    uint32_t err = locate_pixel(y, x, ny, nx, Periodic, p);
+    float dy, dx;
+    locate_op<Periodic>(y, x, g, chk, p.iy0, p.ix0, dy, dx);
+    p.iy1 = p.iy0 + 1;
+    p.ix1 = Periodic ? ((p.ix0 + 1 < g.nx) ? p.ix0 + 1 : 0) : p.ix0 + 1;
     footprint_cells(p, k);
-    if (!active) {
+    if (!active)
         k[0] = k[1] = k[2] = k[3] = NO_CELL;
-        err = 0;
-    }
-    return err;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -140,7 +144,9 @@ preplan_count_kernel(const float *__restrict__ xp, int64_t nsamp, int64_t ncl, i
 
     const int64_t l0 = int64_t(blockIdx.x) * ncl_per_block;
     const int64_t l1 = min(ncl, l0 + ncl_per_block);
-    uint32_t cnt = 0, err = 0;   // cnt is wave-uniform
+    uint32_t cnt = 0;   // wave-uniform
+    const Geom g = make_geom(ny, nx, Periodic);
+    Checks chk;
 
     for (int64_t lb = l0 + 2 * wave; lb < l1; lb += 2 * NW * PP_UNROLL) {
         float y[PP_UNROLL], x[PP_UNROLL];
@@ -156,7 +162,7 @@ preplan_count_kernel(const float *__restrict__ xp, int64_t nsamp, int64_t ncl, i
 #pragma unroll
         for (int u = 0; u < PP_UNROLL; u++) {
             uint32_t k[4];
-            err |= sample_cells<Periodic>(y[u], x[u], ny, nx, act[u], k);
+            sample_cells<Periodic>(y[u], x[u], g, chk, act[u], k);
             uint32_t c, nlo, nhi;
             if (single_cell_halves(k, c, nlo, nhi)) {
                 cnt += nlo + nhi;
@@ -168,7 +174,7 @@ preplan_count_kernel(const float *__restrict__ xp, int64_t nsamp, int64_t ncl, i
         }
     }
 
-    err = wave_reduce_or(err);
+    uint32_t err = wave_reduce_or(chk.bits(g));
     if (lane == 0) {
         s_cnt[wave] = cnt;
         s_err[wave] = err;
@@ -248,6 +254,8 @@ plan_emit_kernel(const float *__restrict__ xp, int64_t nsamp, int64_t ncl, int64
 
     uint32_t err = 0;
     uint32_t running = 0;   // entries written by earlier steps (block-uniform)
+    const Geom g = make_geom(ny, nx, Periodic);
+    Checks chk;
     int parity = 0;
 
     for (int64_t ls = l0; ls < l1; ls += EM_LINES) {
@@ -267,7 +275,7 @@ plan_emit_kernel(const float *__restrict__ xp, int64_t nsamp, int64_t ncl, int64
 #pragma unroll
         for (int u = 0; u < EM_U; u++) {
             uint32_t k[4];
-            err |= sample_cells<Periodic>(y[u], x[u], ny, nx, act[u], k);
+            sample_cells<Periodic>(y[u], x[u], g, chk, act[u], k);
             uint32_t c, nlo, nhi;
             mycell[u] = NO_CELL;
             if (single_cell_halves(k, c, nlo, nhi)) {
@@ -315,7 +323,7 @@ plan_emit_kernel(const float *__restrict__ xp, int64_t nsamp, int64_t ncl, int64
             uint64_t big = __ballot(n > 32);
             if (big) {
                 uint32_t k[4];
-                sample_cells<Periodic>(y[u], x[u], ny, nx, act[u], k);
+                sample_cells<Periodic>(y[u], x[u], g, chk, act[u], k);
                 uint32_t c, nlo = 0, nhi = 0;
                 bool lo, hi;
                 while (extract_one(k, c, lo, hi)) {
@@ -341,7 +349,7 @@ plan_emit_kernel(const float *__restrict__ xp, int64_t nsamp, int64_t ncl, int64
     if (running != limit)
         err |= ERR_INCONSISTENT_NMT;
 
-    err = wave_reduce_or(err);
+    err = wave_reduce_or(err | chk.bits(g));
     if (lane == 0)
         s_err[wave] = err;
     __syncthreads();
