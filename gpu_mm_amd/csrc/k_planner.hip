@@ -109,17 +109,25 @@ __device__ inline bool single_cell_halves(const uint32_t k[4], uint32_t &c, uint
     return __ballot(!one) == 0;
 }
 
-// Footprint cells of one sample (locate_pixel's corners, include/gpu_mm_internals.hpp:124-164,
-// located by locate_op with its range checks deferred to 'chk'). An inactive lane holds NO_CELL
-// everywhere; its y = x = 0 lie in range and leave the checks unchanged.
+// The four corner indices of a sample's bilinear footprint (locate_pixel's,
+// include/gpu_mm_internals.hpp:124-164), located by locate_op with its range checks deferred to
+// 'chk'.
 template<bool Periodic>
-__device__ inline void sample_cells(float y, float x, const Geom &g, Checks &chk, bool active, uint32_t k[4])
+__device__ inline void sample_corners(float y, float x, const Geom &g, Checks &chk, Pixel<float> &p)
 {
-    Pixel<float> p;
     float dy, dx;
     locate_op<Periodic>(y, x, g, chk, p.iy0, p.ix0, dy, dx);
     p.iy1 = p.iy0 + 1;
     p.ix1 = Periodic ? ((p.ix0 + 1 < g.nx) ? p.ix0 + 1 : 0) : p.ix0 + 1;
+}
+
+// Footprint cells of one sample. An inactive lane holds NO_CELL everywhere; its y = x = 0 lie in
+// range and leave the checks unchanged.
+template<bool Periodic>
+__device__ inline void sample_cells(float y, float x, const Geom &g, Checks &chk, bool active, uint32_t k[4])
+{
+    Pixel<float> p;
+    sample_corners<Periodic>(y, x, g, chk, p);
     footprint_cells(p, k);
     if (!active)
         k[0] = k[1] = k[2] = k[3] = NO_CELL;
@@ -598,6 +606,9 @@ qmask_kernel(const uint64_t *__restrict__ mt, int64_t n, const float *__restrict
     __syncthreads();
     const int lane = lane_id(), wave = threadIdx.x / WAVE;
     const int g = lane >> 3, q = lane & 7;
+    // The planner's passes have checked the pointing's range already.
+    const Geom geo = make_geom(ny, nx, Periodic);
+    Checks unchecked;
     uint32_t *list = s_list[wave];
     // Line summary bits, OR-reduced over the entry's 32 samples.
     constexpr uint32_t HAS_L = 1, BAD_L = 2, HAS_Y = 4, BAD_Y = 8, HAS_R = 16, BAD_R = 32, TOUCH_L = 64,
@@ -648,7 +659,7 @@ qmask_kernel(const uint64_t *__restrict__ mt, int64_t n, const float *__restrict
                 if (uint32_t(k) >= nv)
                     continue;
                 Pixel<float> p;
-                locate_pixel(yv[k], xv[k], ny, nx, Periodic, p);
+                sample_corners<Periodic>(yv[k], xv[k], geo, unchecked, p);
                 // Cells of the footprint relative to Y: the owner (p.iy0, p.ix0) is in Y's cell
                 // row iff iy0 >> 6 == cy; its column offset is d, that of the ix1 corner d1.
                 const bool inrow = (p.iy0 >> 6) == cy;
