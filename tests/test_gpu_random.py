@@ -147,3 +147,51 @@ def test_random_batch_parity(gmm, orc, dev, seed):
         b.tod2map(out, debug=True)
         got = out.arr.cpu().numpy()
         assert l1rel(got, ref32) < 1e-6 and l1rel(got, ref64) < F64_BAR, cfg
+
+
+def make_random_instance16(seed):
+    """As make_random_instance, with nt a multiple of 4 (16-bit TODs use 4-sample accesses)."""
+    rng = np.random.default_rng(7000 + seed)
+    periodic = bool(rng.integers(2))
+    ny, nx = int(rng.integers(70, 400)), int(rng.integers(70, 500))
+    ndet, nt = int(rng.integers(1, 7)), 4 * int(rng.integers(25, 750))
+    return random_scan(rng, ndet, nt, ny, nx, periodic), ny, nx, periodic
+
+
+# 16-bit TOD types and the relative rounding of one value (half an ulp at 1).
+TOD16 = [(torch.float16, 2.0 ** -11), (torch.bfloat16, 2.0 ** -8)]
+
+
+@pytest.mark.parametrize("dtype,eps", TOD16, ids=["f16", "bf16"])
+@pytest.mark.parametrize("seed", range(12))
+def test_random_tod16_parity(gmm, orc, dev, seed, dtype, eps):
+    """16-bit TODs on random instances: tod2map of the encoded TOD equals the FP64 oracle's on the
+    decoded samples (1e-5), and map2tod's 16-bit output is the FP64 oracle's to the type's
+    rounding."""
+    xp, ny, nx, periodic = make_random_instance16(seed)
+    ndet, nt = xp.shape[1], xp.shape[2]
+    print(f"\n[random tod16 seed={seed} {dtype}] ndet={ndet} nt={nt} ny={ny} nx={nx} periodic={periodic}",
+          flush=True)
+    xg = torch.from_numpy(xp).to(dev)
+    plan = gmm.PointingPlan(gmm.PointingPrePlan(xg, ny, nx, periodic_xcoord=periodic), xg)
+    lp = gmm.LocalPixelization.make_rectangle(ny, nx, periodic_xcoord=periodic)
+    pix = orc.Pix.rectangle(ny, nx, periodic)
+    rng = np.random.default_rng(100 + seed)
+    tod = rng.standard_normal((ndet, nt)).astype(F32)
+    t16, sc = gmm.encode_tod16(torch.from_numpy(tod).to(dev), dtype)
+    dec = (t16.float() * sc[:, None]).cpu().numpy()
+    ref, err = orc.tod2map(dec.astype(np.float64), xp, pix, f64=True)
+    assert err == 0
+    out = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
+    gmm.tod2map(out, t16, xg, plan, tod_scale=sc, debug=True)
+    assert l1rel(out.arr.cpu().numpy(), ref) < F64_BAR
+
+    # Output scales with headroom for |T + cos Q + sin U| of an N(0,1) map.
+    osc = sc * 8
+    m = rng.standard_normal(3 * lp.npix).astype(F32)
+    ref, _ = orc.map2tod(m.astype(np.float64), xp, pix, f64=True)
+    o16 = torch.empty_like(t16)
+    gmm.map2tod(o16, gmm.LocalMap(lp, torch.from_numpy(m).to(dev)), xg, plan, tod_scale=osc, debug=True)
+    got = (o16.float() * osc[:, None]).cpu().numpy()
+    assert np.all(np.isfinite(got))
+    assert l1rel(got, ref) < eps
