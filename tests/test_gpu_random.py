@@ -1,9 +1,10 @@
 """Randomized GPU parity on small instances: random map sizes, scans back and forth at 0.01-3
 pixels per sample in any direction (drifting and wrapping around in periodic x), detector
 offsets, angles that are constant, slowly or quickly turning or random per sample, and TOD lengths
-that are not multiples of 32 or of 4. The planner (preplan counts, plan bit for bit) and the
-planned operators (FP32 and FP64 oracle bars, device-side invariant checks of the debug mode on)
-against the oracle, as in tests/test_gpu_ops.py."""
+that are not multiples of 32 or of 4. The planner (preplan counts, plan bit for bit), the planned
+operators (FP32 and FP64 oracle bars, device-side invariant checks of the debug mode on; also with
+per-detector gains and FP16/BF16 TODs), the plan-free operators and the batched ones against the
+oracle, as in tests/test_gpu_ops.py."""
 
 import numpy as np
 import pytest
@@ -100,6 +101,27 @@ def test_random_instance_parity(gmm, orc, dev, seed):
     gmm.tod2map(out, torch.from_numpy(d).to(dev), xg, plan, debug=True)
     got = out.arr.cpu().numpy()
     assert l1rel(got, mref32) < 1e-6 and l1rel(got, mref64) < F64_BAR
+
+    # The plan-free kernels on the same inputs.
+    tg.fill_(3.0)
+    gmm.map2tod(tg, mg, xg, None)
+    got = tg.cpu().numpy()
+    assert l1rel(got, ref32) < 1e-6 and l1rel(got, ref64) < F64_BAR
+    out = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
+    gmm.tod2map(out, torch.from_numpy(d).to(dev), xg, None)
+    got = out.arr.cpu().numpy()
+    assert l1rel(got, mref32) < 1e-6 and l1rel(got, mref64) < F64_BAR
+
+    # Per-detector gains (rows of nt samples) against the FP64 response oracle.
+    resp = rng.uniform(0.5, 1.5, (2, ndet)).astype(F32)
+    rg = torch.from_numpy(resp).to(dev)
+    rref, _ = orc.response_map2tod(m, xp, resp, pix)
+    gmm.map2tod(tg, mg, xg, plan, response=rg, debug=True)
+    assert l1rel(tg.cpu().numpy(), rref) < F64_BAR
+    mr, _ = orc.response_tod2map(d, xp, resp, pix)
+    out = gmm.LocalMap(lp, torch.zeros(3 * lp.npix, dtype=torch.float32, device=dev))
+    gmm.tod2map(out, torch.from_numpy(d).to(dev), xg, plan, response=rg, debug=True)
+    assert l1rel(out.arr.cpu().numpy(), mr) < F64_BAR
 
 
 # Work-item shapes of the batched operators: the defaults, many small items, and a persistent
